@@ -1,0 +1,254 @@
+/*
+ * libgrid (MI355X / gfx950 build): public C API for the FFT split-operator hot path.
+ *
+ * Drop-in for the reference library's header for the functions below: same names, argument
+ * order and meaning, same in-place, unnormalised FFT conventions and the same struct field
+ * names that callers read (value, nx/ny/nz, step, x0..z0, kx0..kz0, fft_norm, fft_norm2 /
+ * grid, mass, norm, propagator, boundary).  Compile-time configuration matches the
+ * reference's default build: REAL = double, INT = long, REAL complex = double _Complex
+ * (reference make.conf:13-14, src/structs.h:24-103).
+ *
+ * Execution model: every computational call runs on the GPU.  A grid's device copy is
+ * canonical while it exists; host accessors (cgrid_value_at_index, cgrid_write, ...) copy it
+ * back first.  Code that reads or writes grid->value directly must call cgrid_host_sync()
+ * (read) or cgrid_host_modified() (after writing).  There is no CPU back end: without a
+ * gfx950 device every computational call prints "libgrid: ..." and aborts.
+ */
+
+#ifndef LIBGRID_AMD_GRID_H
+#define LIBGRID_AMD_GRID_H
+
+#include <complex.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include <time.h>
+
+#ifdef __cplusplus
+#error "grid.h is a C99 header (it uses C99 complex types)"
+#endif
+
+/* macros, not typedefs, so that `REAL complex` is a valid type (reference make.conf:62-87) */
+#define REAL double
+#define INT long
+
+#ifndef HBAR
+#define HBAR 1.0 /* reference make.conf:43-44 */
+#endif
+
+#define FMT_R "%le"
+#define FMT_I "%ld"
+
+#define CEXP cexp
+#define CREAL creal
+#define CIMAG cimag
+#define CONJ conj
+#define SQRT sqrt
+#define EXP exp
+#define COS cos
+#define SIN sin
+#define POW pow
+
+#define GRID_EPS 1.0E-16
+
+/* Wavefunction boundary conditions (reference src/defs.h:21-23) */
+#define WF_DIRICHLET_BOUNDARY 1
+#define WF_NEUMANN_BOUNDARY 2
+#define WF_PERIODIC_BOUNDARY 3
+
+/* Wavefunction propagators (reference src/defs.h:26-32).  This build implements
+ * WF_2ND_ORDER_FFT; grid_wf_alloc accepts the others and grid_wf_propagate aborts for them. */
+#define WF_2ND_ORDER_FFT 1
+#define WF_4TH_ORDER_FFT 2
+#define WF_2ND_ORDER_CFFT 3
+#define WF_4TH_ORDER_CFFT 4
+#define WF_2ND_ORDER_CN 5
+#define WF_4TH_ORDER_CN 6
+#define WF_1ST_ORDER_EULER 7
+
+/* Device storage precision of a complex grid (this build's addition) */
+#define GRID_PREC_DOUBLE 0 /* complex128 on the device (default) */
+#define GRID_PREC_SINGLE 1 /* complex64 on the device; FP64 twiddles, phase arguments, reductions */
+
+struct grid_dev;
+
+typedef struct grid_timer_struct {
+  struct timeval zero_time;
+  clock_t zero_clock;
+} grid_timer;
+
+/* Complex grid (reference src/structs.h:24-50).  The FFTW/cuFFT plan fields are absent; the
+ * device state is kept behind `dev`. */
+typedef struct cgrid_struct {
+  REAL complex *value;       /* host copy of the values, [nx][ny][nz], z fastest */
+  char id[32];
+  size_t grid_len;           /* bytes of value */
+  INT nx, ny, nz;
+  REAL step;
+  REAL x0, y0, z0;           /* real space origin */
+  REAL kx0, ky0, kz0;        /* reciprocal space origin (moving frame) */
+  REAL omega;                /* rotation frequency; must be 0 for the FFT propagator */
+  REAL complex (*value_outside)(struct cgrid_struct *grid, INT i, INT j, INT k);
+  void *outside_params_ptr;
+  REAL complex default_outside_params;
+  char host_lock;            /* kept for source compatibility; must stay 0 (no CPU back end) */
+  REAL fft_norm;             /* 1 / (nx ny nz) */
+  REAL fft_norm2;            /* fft_norm * step^d, d = number of dimensions > 1 */
+  char flag;                 /* claim / release */
+  struct grid_dev *dev;      /* device residency (owned by the library) */
+} cgrid;
+
+/* Real grid (reference src/structs.h:57-83); host rows are padded to nz2 = 2 (nz/2 + 1). */
+typedef struct rgrid_struct {
+  REAL *value;
+  char id[32];
+  size_t grid_len;
+  INT nx, ny, nz, nz2;
+  REAL step;
+  REAL x0, y0, z0;
+  REAL kx0, ky0, kz0;
+  REAL (*value_outside)(struct rgrid_struct *grid, INT i, INT j, INT k);
+  void *outside_params_ptr;
+  REAL default_outside_params;
+  char host_lock;
+  REAL fft_norm;
+  REAL fft_norm2;
+  char flag;
+  struct grid_dev *dev;
+} rgrid;
+
+/* Wavefunction (reference src/structs.h:90-103) */
+typedef struct wf_struct {
+  cgrid *grid;
+  REAL mass;
+  REAL norm;                 /* target norm for grid_wf_normalize (default 1) */
+  REAL cfft_width;
+  char boundary;
+  char propagator;
+  cgrid *cworkspace, *cworkspace2, *cworkspace3;
+  REAL (*ts_func)(INT, INT, INT, INT, INT, INT, INT, INT, INT);
+  INT lx, hx, ly, hy, lz, hz;
+  REAL kmax;
+} wf;
+
+/* ---- boundary functions (reference src/cgrid_bc.c:21-86, src/rgrid_bc.c) ---------------- */
+REAL complex cgrid_value_outside_dirichlet(cgrid *grid, INT i, INT j, INT k);
+REAL complex cgrid_value_outside_neumann(cgrid *grid, INT i, INT j, INT k);
+REAL complex cgrid_value_outside_periodic(cgrid *grid, INT i, INT j, INT k);
+REAL rgrid_value_outside_dirichlet(rgrid *grid, INT i, INT j, INT k);
+REAL rgrid_value_outside_neumann(rgrid *grid, INT i, INT j, INT k);
+REAL rgrid_value_outside_periodic(rgrid *grid, INT i, INT j, INT k);
+#define CGRID_DIRICHLET_BOUNDARY cgrid_value_outside_dirichlet
+#define CGRID_NEUMANN_BOUNDARY cgrid_value_outside_neumann
+#define CGRID_PERIODIC_BOUNDARY cgrid_value_outside_periodic
+#define RGRID_DIRICHLET_BOUNDARY rgrid_value_outside_dirichlet
+#define RGRID_NEUMANN_BOUNDARY rgrid_value_outside_neumann
+#define RGRID_PERIODIC_BOUNDARY rgrid_value_outside_periodic
+
+/* ---- runtime ------------------------------------------------------------------------------ */
+void grid_threads_init(INT threads);                      /* reference src/threads.c:89 */
+INT grid_threads(void);                                   /* reference src/threads.c:150 */
+void grid_timer_start(grid_timer *timer);                 /* reference src/ctimer.c:23 */
+REAL grid_timer_wall_clock_time(grid_timer *timer);       /* reference src/ctimer.c:40 */
+REAL grid_timer_cpu_time(grid_timer *timer);              /* reference src/ctimer.c:55 */
+/* GPU on/off; replaces cuda_enable(char, int, int*) (reference src/cuda.c:1168).  One GPU:
+ * ngpus must be 1 (or 0 = default device).  Disabling is refused: there is no CPU back end. */
+void grid_hip_enable(char value, int ngpus, int *gpus);
+char grid_hip_status(void);                               /* replaces cuda_status (src/cuda.c:1195) */
+void grid_hip_sync(void);                                 /* wait for all queued GPU work */
+
+/* ---- complex grids (reference src/cgrid.c) --------------------------------------------- */
+cgrid *cgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL complex (*value_outside)(cgrid *, INT, INT, INT),
+                   void *outside_params_ptr, char *id);                         /* :45 */
+cgrid *cgrid_clone(cgrid *grid, char *id);                                      /* :169 */
+void cgrid_free(cgrid *grid);                                                   /* :332 */
+void cgrid_claim(cgrid *grid);                                                  /* :220 */
+void cgrid_release(cgrid *grid);                                                /* :238 */
+void cgrid_set_origin(cgrid *grid, REAL x0, REAL y0, REAL z0);                 /* :262 */
+void cgrid_shift_origin(cgrid *grid, REAL x0, REAL y0, REAL z0);               /* :280 */
+void cgrid_set_momentum(cgrid *grid, REAL kx0, REAL ky0, REAL kz0);            /* :302 */
+void cgrid_set_rotation(cgrid *grid, REAL omega);                              /* :318 */
+void cgrid_write(cgrid *grid, FILE *out);                                       /* :360 */
+cgrid *cgrid_read(cgrid *grid, FILE *in);                                       /* :382 */
+void cgrid_map(cgrid *grid, REAL complex (*func)(void *arg, REAL x, REAL y, REAL z), void *farg); /* :793 */
+void cgrid_zero(cgrid *grid);
+void cgrid_constant(cgrid *grid, REAL complex c);
+void cgrid_copy(cgrid *dst, cgrid *src);
+void cgrid_multiply(cgrid *grid, REAL complex c);                              /* :1223 */
+REAL cgrid_integral_of_square(cgrid *grid);                                     /* :1610 */
+REAL complex cgrid_integral_of_conjugate_product(cgrid *grida, cgrid *gridb);   /* :1640 */
+void cgrid_fft(cgrid *grid);                                                    /* :1822 */
+void cgrid_inverse_fft(cgrid *grid);                                            /* :1839 */
+void cgrid_scaled_inverse_fft(cgrid *grid, REAL complex c);                     /* :1857 */
+void cgrid_inverse_fft_norm(cgrid *grid);                                       /* :1873 */
+void cgrid_inverse_fft_norm2(cgrid *grid);                                      /* :1887 */
+REAL complex cgrid_value_at_index(cgrid *grid, INT i, INT j, INT k);            /* :1960 */
+void cgrid_value_to_index(cgrid *grid, INT i, INT j, INT k, REAL complex value);
+REAL cgrid_fft_laplace_expectation_value(cgrid *grid);                          /* src/cgrid_diff.c:817 */
+/* device storage precision (GRID_PREC_DOUBLE / GRID_PREC_SINGLE); this build's addition */
+void cgrid_set_precision(cgrid *grid, char prec);
+char cgrid_get_precision(cgrid *grid);
+/* host/device coherence for code that touches grid->value directly (this build's addition) */
+void cgrid_host_sync(cgrid *grid);
+void cgrid_host_modified(cgrid *grid);
+
+/* ---- real grids (reference src/rgrid.c) ------------------------------------------------ */
+rgrid *rgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL (*value_outside)(rgrid *, INT, INT, INT),
+                   void *outside_params_ptr, char *id);                         /* :38 */
+void rgrid_free(rgrid *grid);
+void rgrid_map(rgrid *grid, REAL (*func)(void *arg, REAL x, REAL y, REAL z), void *farg); /* :806 */
+void rgrid_zero(rgrid *grid);
+void rgrid_constant(rgrid *grid, REAL c);
+REAL rgrid_value_at_index(rgrid *grid, INT i, INT j, INT k);
+void rgrid_value_to_index(rgrid *grid, INT i, INT j, INT k, REAL value);
+REAL rgrid_integral(rgrid *grid);
+REAL rgrid_integral_of_square(rgrid *grid);
+void rgrid_host_sync(rgrid *grid);
+void rgrid_host_modified(rgrid *grid);
+
+/* ---- mixed (reference src/ngrid.c) ----------------------------------------------------- */
+REAL grid_grid_expectation_value(cgrid *dgrid, rgrid *opgrid);                  /* :355 */
+void grid_real_to_complex_re(cgrid *dest, rgrid *source);                       /* :53 */
+
+/* ---- wavefunctions (reference src/grid_wf.c, src/grid_wf_fft.c, src/grid_wf_cn.c) ------- */
+wf *grid_wf_alloc(INT nx, INT ny, INT nz, REAL step, REAL mass, char boundary, char propagator, char *id); /* :26 */
+wf *grid_wf_clone(wf *gwf, char *id);                                           /* :131 */
+void grid_wf_free(wf *gwf);                                                     /* :157 */
+void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time);          /* :379 */
+void grid_wf_propagate_potential(wf *gwf, REAL complex tstep, cgrid *potential, REAL cons); /* :569 */
+void grid_wf_propagate_kinetic_fft(wf *gwf, REAL complex time);                /* grid_wf_fft.c:141 */
+void grid_wf_density(wf *gwf, rgrid *density);                                  /* :611 */
+void grid_wf_zero(wf *gwf);
+void grid_wf_constant(wf *gwf, REAL complex c);
+void grid_wf_map(wf *gwf, REAL complex (*func)(void *arg, REAL x, REAL y, REAL z), void *farg);
+REAL grid_wf_norm(wf *gwf);                                                     /* :681 */
+REAL grid_wf_normalize(wf *gwf);                                                /* :694 */
+REAL complex grid_wf_overlap(wf *gwfa, wf *gwfb);
+REAL grid_wf_energy(wf *gwf, rgrid *potential);                                 /* :215 */
+REAL grid_wf_kinetic_energy(wf *gwf);                                           /* :230 */
+REAL grid_wf_potential_energy(wf *gwf, rgrid *potential);                       /* :246 */
+REAL grid_wf_energy_fft(wf *gwf, rgrid *potential);                             /* grid_wf_fft.c:101 */
+REAL grid_wf_kinetic_energy_fft(wf *gwf);                                       /* grid_wf_fft.c:119 */
+REAL grid_wf_energy_cn(wf *gwf, rgrid *potential);                              /* grid_wf_cn.c:19 */
+REAL grid_wf_kinetic_energy_cn(wf *gwf);                                        /* grid_wf_cn.c:37 */
+void grid_wf_analyze_method(char method);                                       /* grid_wf_analyze.c:21 */
+
+/* ---- Gross-Pitaevskii driver (this build's addition; composition of reference calls) ----
+ * One 2nd-order FFT step of i dpsi/dt = (T + vext + g|psi|^2) psi with the nonlinear
+ * potential built from the state at the start of the step, exactly as
+ *   grid_wf_density -> (g * rho + vext) -> grid_real_to_complex_re -> grid_wf_propagate
+ * would, followed by grid_wf_normalize when normalize != 0 — fused into 9 FFT passes.  It
+ * also records E_GP = T + int vext|psi|^2 + (g/2) int |psi|^4 of the start-of-step state. */
+void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g, REAL complex time, char normalize);
+/* E_GP of the state that started the most recent grid_wf_gpe_propagate call. */
+REAL grid_wf_gpe_last_energy(wf *gwf);
+/* E_GP of the current state (FFT kinetic energy, no k0 shift, as grid_wf_kinetic_energy_fft). */
+REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g);
+/* Imaginary-time relaxation with renormalisation every step until |E_n - E_{n-1}| < tol or
+ * max_steps; returns the number of steps taken and stores the last energy in *energy. */
+INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_steps, REAL *energy);
+
+#endif

@@ -1,0 +1,167 @@
+/*
+ * Thin C-ABI between libgrid's C host code and its hand-written HIP kernels (gfx950).
+ *
+ * Everything here takes plain device pointers, sizes and scalars; no C99 complex, no HIP
+ * types, no torch types.  The C host library (grid.h API) is the only intended caller; the
+ * functions are exported from libgrid.so so that tests and the benchmark can time them.
+ *
+ * All launches go to one HIP stream per process (created by hgrid_init at default priority);
+ * every call below is asynchronous unless it says otherwise.  A non-zero return value means
+ * the HIP runtime reported an error; hgrid_last_error() describes it.
+ *
+ * It replaces the reference's "char hook -> extern C ...W launcher" seam
+ * (src/grid_wf-cuda.c:18-49 -> src/grid_wf-cuda2.cu:72-104, src/cgrid_fft-cuda.c:41-174,
+ * src/cgrid-cuda2.cu:1187-1220), but with device-canonical grids and fused FFT passes.
+ */
+
+#ifndef LIBGRID_AMD_GRID_HIP_H
+#define LIBGRID_AMD_GRID_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ---------------------------------------------------------------------------- */
+
+/* Select the device and create the library stream.  Returns 0 on success, -1 if no HIP device
+ * is present, -2 if the device is not gfx950 (the code objects are built for gfx950 only). */
+int hgrid_init(int device);
+int hgrid_initialized(void);
+const char *hgrid_last_error(void);
+const char *hgrid_device_name(void);          /* gcnArchName of the selected device */
+int hgrid_malloc(void **ptr, size_t bytes);
+int hgrid_free(void *ptr);
+int hgrid_memcpy_h2d(void *dst, const void *src, size_t bytes);   /* synchronous */
+int hgrid_memcpy_d2h(void *dst, const void *src, size_t bytes);   /* synchronous */
+int hgrid_memcpy_d2d(void *dst, const void *src, size_t bytes);   /* async, library stream */
+int hgrid_memset(void *ptr, int value, size_t bytes);             /* async fill kernel */
+int hgrid_sync(void);                                              /* wait for the library stream */
+/* Elapsed time of the library stream between two marks (HIP events recorded on that stream). */
+int hgrid_event_create(void **ev);
+int hgrid_event_destroy(void *ev);
+int hgrid_event_record(void *ev);
+int hgrid_event_elapsed_ms(void *start, void *stop, float *ms);   /* synchronizes on stop */
+int hgrid_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- storage conversion ----------------------------------------------------------------- */
+
+/* FP64 complex <-> FP32 complex (device arrays of n elements).  Reduced-precision storage
+ * keeps a host complex128 mirror; these convert on upload / download. */
+int hgrid_c128_to_c64(void *dst64, const void *src128, size_t n);
+int hgrid_c64_to_c128(void *dst128, const void *src64, size_t n);
+/* Pack an nz2-padded real rgrid row layout into a dense [nx*ny][nz] array (and back). */
+int hgrid_rpack(double *dst, const double *src, long rows, long nz, long nz2);
+int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
+
+/* ---- FFT passes ------------------------------------------------------------------------- */
+
+/* Precision of a device complex array. */
+#define HGRID_C128 0   /* double2 */
+#define HGRID_C64  1   /* float2 (twiddles and phase tables are computed in FP64, stored FP32) */
+
+/* Pass flags (bit set).  Stage order inside one pass:
+ *   load [* *scale] -> PRE ops -> FFT #1 -> MID ops -> [FFT #2] -> * post -> POST ops -> store */
+#define HGRID_PF_PRESCALE   0x0001u  /* multiply loads by the complex device scalar *scale */
+#define HGRID_PF_PRE_RHO    0x0002u  /* pre: rho_out = |psi|^2 (GPE density of the input state) */
+#define HGRID_PF_PRE_RED    0x0004u  /* pre: partial sums of |psi|^2, vext|psi|^2, |psi|^4 */
+#define HGRID_PF_MID_KIN    0x0008u  /* mid: psi *= ktab_a[n] * ktab_1[c1] * ktab_2[c2] */
+#define HGRID_PF_MID_KRED   0x0010u  /* mid: partial sum of k^2 |psi_k|^2 (before MID_KIN) */
+#define HGRID_PF_MID_POT    0x0020u  /* mid: psi *= mid * exp(c * (cons + pot)), complex pot */
+#define HGRID_PF_MID_GPE    0x0040u  /* mid: psi *= mid * exp(c * (cons + vext + g * rho)) */
+#define HGRID_PF_POST_RED   0x0080u  /* post: partial sum of |psi|^2 of the stored values */
+#define HGRID_PF_NOSTORE    0x0100u  /* reduce only; do not write the array */
+
+/* Reduction channels (partials[chan * nblocks + block]). */
+#define HGRID_RED_NORM   0   /* sum |psi|^2 of the pass input   (PRE_RED) */
+#define HGRID_RED_VEXT   1   /* sum vext |psi|^2                 (PRE_RED, needs vext) */
+#define HGRID_RED_QUART  2   /* sum |psi|^4                      (PRE_RED) */
+#define HGRID_RED_KIN    3   /* sum k^2 |psi_k|^2                (MID_KRED) */
+#define HGRID_RED_POST   4   /* sum |psi|^2 of the pass output   (POST_RED) */
+#define HGRID_RED_NCHAN  5
+
+typedef struct hgrid_pass {
+  void *in;              /* device array (complex, precision prec) */
+  void *out;             /* may equal in (in-place) */
+  int prec;              /* HGRID_C128 / HGRID_C64 */
+  long n0, n1, n2;       /* squeezed shape; n2 is contiguous */
+  int axis;              /* 0, 1 or 2 */
+  int nfft;              /* 1 or 2 transforms along the axis in this pass */
+  int dir1, dir2;        /* -1 forward (exp(-i..)), +1 inverse (exp(+i..)), unnormalised */
+  unsigned flags;
+  const void *tw;        /* W_L^m = exp(-2 pi i m / L), m < L, precision prec, L = length of axis */
+  const double *scale;   /* PRESCALE: device complex scalar {re, im} */
+  double post;           /* factor applied to the stored values (1.0 = none) */
+  double mid;            /* factor applied at MID_POT / MID_GPE / MID_SCALE */
+  /* kinetic tables (precision prec): factor for the element index along axis, and for the two
+   * other squeezed coordinates c1, c2 of the pencil (see hgrid_pass_coords) */
+  const void *ktab_a, *ktab_1, *ktab_2;
+  const double *k2_a, *k2_1, *k2_2;   /* |k|^2 per axis for MID_KRED (no k0 shift) */
+  /* potential */
+  const void *pot;       /* complex potential, same layout and precision as in */
+  const double *vext;    /* dense real external potential (FP64) */
+  const double *rho;     /* dense real density (FP64) for MID_GPE */
+  double g;              /* nonlinear coupling */
+  double c_re, c_im;     /* c = -i * tstep / hbar */
+  double cons;           /* constant added to the potential */
+  double *rho_out;       /* PRE_RHO destination (FP64, dense) */
+  double *partials;      /* HGRID_RED_NCHAN * nblocks doubles (see hgrid_pass_blocks) */
+} hgrid_pass;
+
+/* Number of workgroups a pass launches (size of each partials channel). */
+long hgrid_pass_blocks(int prec, long n0, long n1, long n2, int axis);
+/* Launch one pass. Returns 0, or -1 for an unsupported axis length / precision. */
+int hgrid_fft_pass(const hgrid_pass *p);
+/* Axis lengths the pass kernels support (power of two, 1..1024). */
+int hgrid_fft_supported(long n);
+
+/* ---- reductions ------------------------------------------------------------------------- */
+
+/* out[c] = sum_{b < nblocks} partials[c * nblocks + b] for c < nchan, deterministic order,
+ * FP64 Neumaier-compensated.  out is a device array. */
+int hgrid_reduce_partials(double *out, const double *partials, long nblocks, int nchan);
+
+/* Elementwise reductions over device arrays (n elements), written to the device scalar *out
+ * (deterministic; scratch must hold hgrid_reduce_scratch(n) doubles). */
+long hgrid_reduce_scratch(long n);
+int hgrid_sum_norm2(double *out, const void *a, int prec, long n, const double *scale, double *scratch);
+int hgrid_sum_vnorm2(double *out, const void *a, int prec, const double *v, long n, const double *scale, double *scratch);
+int hgrid_sum_conj_product(double *out2, const void *a, const void *b, int prec, long n, double *scratch);
+/* sum (k2x[i] + k2y[j] + k2z[k]) |a[i,j,k]|^2 over an nx x ny x nz array */
+int hgrid_sum_k2norm2(double *out, const void *a, int prec, long nx, long ny, long nz, const double *k2x,
+                      const double *k2y, const double *k2z, const double *scale, double *scratch);
+/* sum a[i] (squared = 0) or sum a[i]^2 (squared = 1) of a dense real array */
+int hgrid_sum_real(double *out, const double *a, long n, int squared, double *scratch);
+
+/* ---- pointwise kernels ------------------------------------------------------------------ */
+
+/* psi *= exp(c * (cons + pot)) with c = (c_re, c_im); pot complex (same precision). */
+int hgrid_potential_phase(void *psi, const void *pot, int prec, long n, double c_re, double c_im, double cons,
+                          const double *scale);
+/* psi[i,j,k] *= ta[i] * tb[j] * tc[k]  (3 tables of complex factors, precision prec) */
+int hgrid_kinetic_phase(void *psi, int prec, long nx, long ny, long nz, const void *ta, const void *tb,
+                        const void *tc, const double *scale);
+/* psi *= (c_re + i c_im) * (*scale if scale) */
+int hgrid_cscale(void *psi, int prec, long n, double c_re, double c_im, const double *scale);
+/* dst = |src|^2 (dense FP64 output) */
+int hgrid_density(double *dst, const void *src, int prec, long n, const double *scale);
+/* dst = (complex) src (real part, imaginary zero) */
+int hgrid_real_to_complex(void *dst, int prec, const double *src, long n);
+/* dst = c (complex constant) */
+int hgrid_cfill(void *dst, int prec, long n, double c_re, double c_im);
+/* device complex scalar ops on a 2-double slot: s = {re, im} */
+int hgrid_scalar_set(double *s, double re, double im);
+int hgrid_scalar_mul(double *s, double re, double im);
+/* s = sqrt(target / (sum[0] * dv)) (renormalisation factor from a device sum) */
+int hgrid_scalar_renorm(double *s, const double *sum, double target, double dv);
+/* FD kinetic energy integrand: out = sum Re(conj(psi) * laplace7(psi)) / h^2 (periodic,
+ * Neumann or Dirichlet boundary via bc = 2/1/0). */
+int hgrid_fd_laplace_expect(double *out, const void *psi, int prec, long nx, long ny, long nz, int bc,
+                            const double *scale, double *scratch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

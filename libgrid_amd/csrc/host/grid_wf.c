@@ -1,0 +1,545 @@
+/*
+ * Wavefunctions: allocation, the 2nd-order FFT split-operator step, the potential and
+ * kinetic phases, norm, energies and the fused Gross-Pitaevskii step.
+ *
+ * grid_wf_propagate (reference src/grid_wf.c:379-432, branch WF_2ND_ORDER_FFT without an
+ * absorbing boundary) computes
+ *     psi <- IFFT(K FFT(psi)) / N ; psi <- psi exp(-i t (cons + V) / hbar) ; psi <- IFFT(K FFT(psi)) / N
+ * with K = exp(-i (t/2) hbar k^2 / 2m) and cons = -(hbar^2 / 2m) |k0|^2, in that order.  For
+ * a grid with at least two non-trivial axes it runs as 2 * m + 3 pencil passes (9 for 3-D):
+ * the kinetic factor sits between a forward and an inverse transform of the outermost axis,
+ * the potential factor between an inverse and a forward transform of the contiguous axis,
+ * and the 1/N of both normalised inverse transforms is folded into the kinetic tables.
+ */
+#include "internal.h"
+
+char grid_analyze_method = (char)-1; /* 0 = finite difference, 1 = FFT, -1 = not set */
+
+void grid_wf_analyze_method(char method) { grid_analyze_method = method; }
+
+wf *grid_wf_alloc(INT nx, INT ny, INT nz, REAL step, REAL mass, char boundary, char propagator, char *id) {
+  wf *gwf;
+  REAL complex (*value_outside)(struct cgrid_struct *grid, INT i, INT j, INT k) = NULL;
+  if (boundary < WF_DIRICHLET_BOUNDARY || boundary > WF_PERIODIC_BOUNDARY) {
+    fprintf(stderr, "libgrid: Error in grid_wf_alloc(). Unknown boundary condition.\n");
+    return NULL;
+  }
+  if (propagator < WF_2ND_ORDER_FFT || propagator > WF_1ST_ORDER_EULER) {
+    fprintf(stderr, "libgrid: Error in grid_wf_alloc(). Unknown propagator.\n");
+    return NULL;
+  }
+  if (boundary == WF_DIRICHLET_BOUNDARY && propagator < WF_2ND_ORDER_CN) {
+    fprintf(stderr, "libgrid: Error in grid_wf_alloc(). Invalid boundary condition - propagator combination. Dirichlet "
+                    "condition can only be used with Crank-Nicolson propagator.\n");
+    return NULL;
+  }
+  if (!(gwf = (wf *)calloc(1, sizeof(wf)))) {
+    fprintf(stderr, "libgrid: Error in grid_wf_alloc(). Could not allocate memory for wf.\n");
+    return NULL;
+  }
+  switch (boundary) {
+    case WF_DIRICHLET_BOUNDARY: value_outside = CGRID_DIRICHLET_BOUNDARY; break;
+    case WF_NEUMANN_BOUNDARY: value_outside = CGRID_NEUMANN_BOUNDARY; break;
+    case WF_PERIODIC_BOUNDARY: value_outside = CGRID_PERIODIC_BOUNDARY; break;
+  }
+  gwf->grid = cgrid_alloc(nx, ny, nz, step, value_outside, 0, id);
+  if (grid_analyze_method == -1) grid_analyze_method = 0; /* reference src/cgrid.c:141-149 */
+  gwf->mass = mass;
+  gwf->norm = 1.0;
+  gwf->boundary = boundary;
+  gwf->propagator = propagator;
+  gwf->cworkspace = gwf->cworkspace2 = gwf->cworkspace3 = NULL;
+  gwf->ts_func = NULL;
+  gwf->cfft_width = 1.0;
+  gwf->kmax = 9999.0;
+  return gwf;
+}
+
+wf *grid_wf_clone(wf *gwf, char *id) {
+  wf *nwf = (wf *)malloc(sizeof(wf));
+  if (!nwf) lg_fatal("Out of memory in grid_wf_clone().");
+  memcpy(nwf, gwf, sizeof(wf));
+  nwf->grid = cgrid_clone(gwf->grid, id);
+  nwf->cworkspace = nwf->cworkspace2 = nwf->cworkspace3 = NULL;
+  return nwf;
+}
+
+void grid_wf_free(wf *gwf) {
+  if (!gwf) return;
+  if (gwf->grid) cgrid_free(gwf->grid);
+  if (gwf->cworkspace) cgrid_free(gwf->cworkspace);
+  if (gwf->cworkspace2) cgrid_free(gwf->cworkspace2);
+  if (gwf->cworkspace3) cgrid_free(gwf->cworkspace3);
+  free(gwf);
+}
+
+void grid_wf_zero(wf *gwf) { cgrid_zero(gwf->grid); }
+void grid_wf_constant(wf *gwf, REAL complex c) { cgrid_constant(gwf->grid, c); }
+void grid_wf_map(wf *gwf, REAL complex (*func)(void *arg, REAL x, REAL y, REAL z), void *farg) {
+  cgrid_map(gwf->grid, func, farg);
+}
+
+REAL grid_wf_norm(wf *gwf) { return cgrid_integral_of_square(gwf->grid); }
+
+REAL grid_wf_normalize(wf *gwf) {
+  REAL norm = grid_wf_norm(gwf);
+  cgrid_multiply(gwf->grid, SQRT(gwf->norm / norm));
+  return norm;
+}
+
+REAL complex grid_wf_overlap(wf *gwfa, wf *gwfb) { return cgrid_integral_of_conjugate_product(gwfa->grid, gwfb->grid); }
+
+/* ---- pointwise operators ------------------------------------------------------------------ */
+
+void grid_wf_propagate_potential(wf *gwf, REAL complex tstep, cgrid *potential, REAL cons) {
+  if (!potential) return;
+  cgrid *g = gwf->grid;
+  if (potential->nx != g->nx || potential->ny != g->ny || potential->nz != g->nz)
+    lg_fatal("grid_wf_propagate_potential: potential and wavefunction grids differ in size");
+  if (gwf->ts_func) lg_fatal("grid_wf_propagate_potential: absorbing boundaries are not supported by this build");
+  if (potential->dev->prec != g->dev->prec) cgrid_set_precision(potential, cgrid_get_precision(g));
+  const void *pot = lg_c_in_flushed(potential);
+  const double *scale;
+  void *psi = lg_c_inout(g, &scale);
+  /* c = -i tstep / hbar */
+  const double c_re = cimag(tstep) / HBAR, c_im = -creal(tstep) / HBAR;
+  lg_hip(hgrid_potential_phase(psi, pot, g->dev->prec, (long)g->dev->n, c_re, c_im, cons, scale), "potential_phase");
+  lg_c_written(g, 1);
+}
+
+void grid_wf_propagate_kinetic_fft(wf *gwf, REAL complex time) {
+  cgrid *g = gwf->grid;
+  const REAL complex tm = -I * time * HBAR / (gwf->mass * 2.0); /* reference src/grid_wf_fft.c:146 */
+  const double *scale;
+  void *psi = lg_c_inout(g, &scale);
+  ktab_cache *k = lg_kinetic_tables(g, creal(tm), cimag(tm), 1.0);
+  lg_hip(hgrid_kinetic_phase(psi, g->dev->prec, g->nx, g->ny, g->nz, k->t[0], k->t[1], k->t[2], scale), "kinetic_phase");
+  lg_c_written(g, 1);
+}
+
+void grid_wf_density(wf *gwf, rgrid *density) {
+  cgrid *g = gwf->grid;
+  if (density->nx != g->nx || density->ny != g->ny || density->nz != g->nz) lg_fatal("grid_wf_density: size mismatch");
+  const double *scale;
+  const void *psi = lg_c_in(g, &scale);
+  double *out = lg_r_out(density);
+  lg_hip(hgrid_density(out, psi, g->dev->prec, (long)g->dev->n, scale), "density");
+  lg_r_written(density);
+}
+
+void grid_real_to_complex_re(cgrid *dest, rgrid *source) {
+  if (dest->nx != source->nx || dest->ny != source->ny || dest->nz != source->nz)
+    lg_fatal("grid_real_to_complex_re: size mismatch");
+  dest->step = source->step;
+  const double *src = lg_r_in(source);
+  void *dst = lg_c_out(dest);
+  lg_hip(hgrid_real_to_complex(dst, dest->dev->prec, src, (long)dest->dev->n), "real_to_complex");
+  lg_c_written(dest, 1);
+}
+
+static double fetch(const double *dptr) {
+  double v;
+  lg_hip(hgrid_memcpy_d2h(&v, dptr, sizeof(double)), "hipMemcpy D2H (scalar)");
+  return v;
+}
+
+REAL grid_grid_expectation_value(cgrid *dgrid, rgrid *opgrid) {
+  if (dgrid->nx != opgrid->nx || dgrid->ny != opgrid->ny || dgrid->nz != opgrid->nz)
+    lg_fatal("grid_grid_expectation_value: size mismatch");
+  const double *scale;
+  const void *psi = lg_c_in(dgrid, &scale);
+  const double *v = lg_r_in(opgrid);
+  grid_dev *d = dgrid->dev;
+  long n = (long)d->n;
+  double *scr = lg_partials(hgrid_reduce_scratch(n));
+  lg_hip(hgrid_sum_vnorm2(d->dscal + GS_TMP, psi, d->prec, v, n, scale, scr), "sum_vnorm2");
+  return fetch(d->dscal + GS_TMP) * lg_step_factor(dgrid->nx, dgrid->ny, dgrid->nz, dgrid->step);
+}
+
+/* ---- pass plumbing ------------------------------------------------------------------------ */
+
+/* table pointers of squeezed axis a (trivial / padding axes use a trivial physical axis,
+ * whose table holds the single value 1 or 0) */
+static int trivial_phys(const cgrid *g) { return g->nx == 1 ? 0 : g->ny == 1 ? 1 : 2; }
+
+static const void *ktab_sq(const ktab_cache *k, const lg_shape *s, const cgrid *g, int a) {
+  return k->t[s->phys[a] >= 0 ? s->phys[a] : trivial_phys(g)];
+}
+static const double *k2_sq(const ktab_cache *k, const lg_shape *s, const cgrid *g, int a) {
+  return k->k2[s->phys[a] >= 0 ? s->phys[a] : trivial_phys(g)];
+}
+
+/* the two other squeezed axes (c1, c2) of a pass on `axis`, as the kernels define them */
+static void other_axes(int axis, int *a1, int *a2) {
+  if (axis == 0) { *a1 = 1; *a2 = 2; }
+  else if (axis == 1) { *a1 = 0; *a2 = 2; }
+  else { *a1 = 0; *a2 = 1; }
+}
+
+static void set_kin(hgrid_pass *p, const ktab_cache *k, const lg_shape *s, const cgrid *g) {
+  int a1, a2;
+  other_axes(p->axis, &a1, &a2);
+  p->ktab_a = ktab_sq(k, s, g, p->axis);
+  p->ktab_1 = ktab_sq(k, s, g, a1);
+  p->ktab_2 = ktab_sq(k, s, g, a2);
+}
+
+static void set_k2(hgrid_pass *p, const ktab_cache *k, const lg_shape *s, const cgrid *g) {
+  int a1, a2;
+  other_axes(p->axis, &a1, &a2);
+  p->k2_a = k2_sq(k, s, g, p->axis);
+  p->k2_1 = k2_sq(k, s, g, a1);
+  p->k2_2 = k2_sq(k, s, g, a2);
+}
+
+static void *wf_work(cgrid *g) {
+  grid_dev *d = g->dev;
+  if (!d->work) lg_hip(hgrid_malloc(&d->work, d->n * lg_prec_size(d->prec)), "hipMalloc (wf workspace)");
+  return d->work;
+}
+
+/* ---- fused step --------------------------------------------------------------------------- */
+
+typedef struct {
+  int gpe;               /* 0 complex potential (or none), 1 vext + g rho */
+  const void *pot;       /* complex potential on the device (NULL = no potential) */
+  const double *vext;
+  double g;
+  double c_re, c_im, cons;
+  double tm_re, tm_im;   /* time_mass of the half step */
+  int want_energy;       /* reductions for E of the input state */
+  int want_post_norm;    /* sum |psi|^2 of the output */
+} step_args;
+
+static void fused_step(wf *gwf, const step_args *sa) {
+  cgrid *g = gwf->grid;
+  grid_dev *d = g->dev;
+  lg_shape s;
+  lg_squeeze(g->nx, g->ny, g->nz, &s);
+  const double *scale;
+  void *psi = lg_c_inout(g, &scale);
+  const int prec = d->prec, aK = 3 - s.m;
+  ktab_cache *kt = lg_kinetic_tables(g, sa->tm_re, sa->tm_im, g->fft_norm);
+  ktab_cache *k2 = (sa->want_energy) ? lg_k2_tables(g) : NULL;
+  if (sa->gpe && !d->rho) lg_hip(hgrid_malloc((void **)&d->rho, d->n * sizeof(double)), "hipMalloc (density)");
+
+  const long nb2 = hgrid_pass_blocks(prec, s.n[0], s.n[1], s.n[2], 2);
+  const long nbK = hgrid_pass_blocks(prec, s.n[0], s.n[1], s.n[2], aK);
+  double *part = lg_partials(HGRID_RED_NCHAN * (2 * nb2 + nbK));
+  double *part1 = part, *partK = part + HGRID_RED_NCHAN * nb2, *part9 = partK + HGRID_RED_NCHAN * nbK;
+  hgrid_pass p;
+
+  /* P1: forward along the contiguous axis; pending scale, density and input reductions */
+  lg_pass_init(&p, &s, prec, psi, psi, 2);
+  if (scale) { p.flags |= HGRID_PF_PRESCALE; p.scale = scale; }
+  if (sa->gpe) { p.flags |= HGRID_PF_PRE_RHO; p.rho_out = d->rho; }
+  if (sa->want_energy) { p.flags |= HGRID_PF_PRE_RED; p.vext = sa->vext; p.partials = part1; }
+  lg_run_pass(&p);
+  if (sa->want_energy) lg_hip(hgrid_reduce_partials(d->dscal + GS_RED, part1, nb2, 3), "reduce");
+
+  for (int half = 0; half < 2; ++half) {
+    for (int axis = 1; axis > aK; --axis) {
+      lg_pass_init(&p, &s, prec, psi, psi, axis);
+      lg_run_pass(&p);
+    }
+    /* kinetic: forward, K / N, inverse along the outermost axis */
+    lg_pass_init(&p, &s, prec, psi, psi, aK);
+    p.nfft = 2;
+    p.dir1 = -1;
+    p.dir2 = +1;
+    p.flags |= HGRID_PF_MID_KIN;
+    set_kin(&p, kt, &s, g);
+    if (half == 0 && sa->want_energy) {
+      p.flags |= HGRID_PF_MID_KRED;
+      set_k2(&p, k2, &s, g);
+      p.partials = partK;
+    }
+    lg_run_pass(&p);
+    if (half == 0 && sa->want_energy)
+      lg_hip(hgrid_reduce_partials(d->dscal + GS_RED + HGRID_RED_KIN, partK + HGRID_RED_KIN * nbK, nbK, 1), "reduce");
+    for (int axis = aK + 1; axis <= 1; ++axis) {
+      lg_pass_init(&p, &s, prec, psi, psi, axis);
+      p.dir1 = +1;
+      lg_run_pass(&p);
+    }
+    if (half == 0) {
+      /* potential: inverse, exp(c (cons + V)), forward along the contiguous axis */
+      if (sa->gpe || sa->pot) {
+        lg_pass_init(&p, &s, prec, psi, psi, 2);
+        p.nfft = 2;
+        p.dir1 = +1;
+        p.dir2 = -1;
+        p.c_re = sa->c_re;
+        p.c_im = sa->c_im;
+        p.cons = sa->cons;
+        p.mid = 1.0;
+        if (sa->gpe) {
+          p.flags |= HGRID_PF_MID_GPE;
+          p.vext = sa->vext;
+          p.rho = d->rho;
+          p.g = sa->g;
+        } else {
+          p.flags |= HGRID_PF_MID_POT;
+          p.pot = sa->pot;
+        }
+        lg_run_pass(&p);
+      } else {
+        lg_pass_init(&p, &s, prec, psi, psi, 2);
+        p.dir1 = +1;
+        lg_run_pass(&p);
+        lg_pass_init(&p, &s, prec, psi, psi, 2);
+        p.dir1 = -1;
+        lg_run_pass(&p);
+      }
+    }
+  }
+  /* last: inverse along the contiguous axis (1/N already applied with K) */
+  lg_pass_init(&p, &s, prec, psi, psi, 2);
+  p.dir1 = +1;
+  if (sa->want_post_norm) { p.flags |= HGRID_PF_POST_RED; p.partials = part9; }
+  lg_run_pass(&p);
+  if (sa->want_post_norm)
+    lg_hip(hgrid_reduce_partials(d->dscal + GS_RED + HGRID_RED_POST, part9 + HGRID_RED_POST * nb2, nb2, 1), "reduce");
+  lg_c_written(g, 1);
+}
+
+/* unfused sequence for grids with fewer than two non-trivial axes (reference order) */
+static void split_step_unfused(wf *gwf, cgrid *potential, REAL complex time, REAL cons) {
+  cgrid *g = gwf->grid;
+  cgrid_fft(g);
+  grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
+  cgrid_inverse_fft_norm(g);
+  grid_wf_propagate_potential(gwf, time, potential, cons);
+  cgrid_fft(g);
+  grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
+  cgrid_inverse_fft_norm(g);
+}
+
+static void check_fft_grid(cgrid *g) {
+  INT d[3] = {g->nx, g->ny, g->nz};
+  for (int a = 0; a < 3; ++a)
+    if (d[a] > 1 && !hgrid_fft_supported(d[a]))
+      lg_fatal("FFT length %ld is not supported (powers of two up to 1024)", (long)d[a]);
+}
+
+void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
+  cgrid *g = gwf->grid;
+  const REAL kx0 = g->kx0, ky0 = g->ky0, kz0 = g->kz0;
+  const REAL cons = -(HBAR * HBAR / (2.0 * gwf->mass)) * (kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
+  if (gwf->propagator != WF_2ND_ORDER_FFT)
+    lg_fatal("Error in grid_wf_propagate(). Only WF_2ND_ORDER_FFT is implemented by this build (propagator %d).",
+             (int)gwf->propagator);
+  if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
+  if (gwf->ts_func) lg_fatal("absorbing boundaries (grid_wf_boundary) are not supported by this build");
+  check_fft_grid(g);
+  lg_shape s;
+  lg_squeeze(g->nx, g->ny, g->nz, &s);
+  if (s.m < 2) {
+    split_step_unfused(gwf, potential, time, cons);
+    return;
+  }
+  step_args sa;
+  memset(&sa, 0, sizeof(sa));
+  if (potential) {
+    if (potential->nx != g->nx || potential->ny != g->ny || potential->nz != g->nz)
+      lg_fatal("grid_wf_propagate: potential and wavefunction grids differ in size");
+    if (potential->dev->prec != g->dev->prec) cgrid_set_precision(potential, cgrid_get_precision(g));
+    sa.pot = lg_c_in_flushed(potential);
+  }
+  sa.c_re = cimag(time) / HBAR;
+  sa.c_im = -creal(time) / HBAR;
+  sa.cons = cons;
+  const REAL complex tm = -I * (0.5 * time) * HBAR / (gwf->mass * 2.0);
+  sa.tm_re = creal(tm);
+  sa.tm_im = cimag(tm);
+  fused_step(gwf, &sa);
+}
+
+/* ---- energies ----------------------------------------------------------------------------- */
+
+/* sum k^2 |FFT(psi)|^2 without modifying psi: forward passes into the workspace, the last one
+ * reduces instead of storing. */
+static double sum_k2_spectrum(cgrid *g) {
+  grid_dev *d = g->dev;
+  lg_shape s;
+  lg_squeeze(g->nx, g->ny, g->nz, &s);
+  check_fft_grid(g);
+  if (s.m == 0) return 0.0;
+  const double *scale;
+  void *psi = lg_c_in(g, &scale);
+  void *work = wf_work(g);
+  ktab_cache *k2 = lg_k2_tables(g);
+  const int aK = 3 - s.m;
+  const long nbK = hgrid_pass_blocks(d->prec, s.n[0], s.n[1], s.n[2], aK);
+  double *part = lg_partials(HGRID_RED_NCHAN * nbK);
+  hgrid_pass p;
+  int first = 1;
+  for (int axis = 2; axis >= aK; --axis) {
+    lg_pass_init(&p, &s, d->prec, first ? psi : work, work, axis);
+    if (first && scale) { p.flags |= HGRID_PF_PRESCALE; p.scale = scale; }
+    if (axis == aK) {
+      p.flags |= HGRID_PF_MID_KRED | HGRID_PF_NOSTORE;
+      set_k2(&p, k2, &s, g);
+      p.partials = part;
+    }
+    lg_run_pass(&p);
+    first = 0;
+  }
+  lg_hip(hgrid_reduce_partials(d->dscal + GS_TMP, part + HGRID_RED_KIN * nbK, nbK, 1), "reduce");
+  return fetch(d->dscal + GS_TMP);
+}
+
+REAL grid_wf_kinetic_energy_fft(wf *gwf) {
+  cgrid *g = gwf->grid;
+  /* -hbar^2/2m * laplace expectation = hbar^2/2m * sum k^2 |psi_k|^2 * fft_norm2 */
+  return HBAR * HBAR / (2.0 * gwf->mass) * sum_k2_spectrum(g) * g->fft_norm2;
+}
+
+REAL grid_wf_potential_energy(wf *gwf, rgrid *potential) { return grid_grid_expectation_value(gwf->grid, potential); }
+
+REAL grid_wf_energy_fft(wf *gwf, rgrid *potential) {
+  REAL en = grid_wf_kinetic_energy_fft(gwf);
+  if (potential) en += grid_wf_potential_energy(gwf, potential);
+  return en;
+}
+
+static int bc_code(cgrid *g) {
+  if (g->value_outside == CGRID_PERIODIC_BOUNDARY) return 2;
+  if (g->value_outside == CGRID_NEUMANN_BOUNDARY) return 1;
+  if (g->value_outside == CGRID_DIRICHLET_BOUNDARY) {
+    if (*((REAL complex *)g->outside_params_ptr) != 0.0) lg_fatal("non-zero Dirichlet boundary value is not supported");
+    return 0;
+  }
+  lg_fatal("user-defined boundary functions are not supported on the GPU");
+}
+
+REAL grid_wf_kinetic_energy_cn(wf *gwf) {
+  cgrid *g = gwf->grid;
+  const double *scale;
+  const void *psi = lg_c_in(g, &scale);
+  grid_dev *d = g->dev;
+  double *scr = lg_partials(hgrid_reduce_scratch((long)d->n));
+  lg_hip(hgrid_fd_laplace_expect(d->dscal + GS_TMP, psi, d->prec, g->nx, g->ny, g->nz, bc_code(g), scale, scr),
+         "fd_laplace_expect");
+  /* Re int psi^* (-hbar^2/2m) laplace(psi) (reference src/grid_wf_cn.c:37-49) */
+  return -HBAR * HBAR / (2.0 * gwf->mass) * fetch(d->dscal + GS_TMP) / (g->step * g->step) *
+         lg_step_factor(g->nx, g->ny, g->nz, g->step);
+}
+
+REAL grid_wf_energy_cn(wf *gwf, rgrid *potential) {
+  REAL en = grid_wf_kinetic_energy_cn(gwf);
+  if (potential) en += grid_wf_potential_energy(gwf, potential);
+  return en;
+}
+
+REAL grid_wf_energy(wf *gwf, rgrid *potential) {
+  if (gwf->propagator == WF_2ND_ORDER_CN || gwf->propagator == WF_4TH_ORDER_CN || !grid_analyze_method)
+    return grid_wf_energy_cn(gwf, potential);
+  return grid_wf_energy_fft(gwf, potential);
+}
+
+REAL grid_wf_kinetic_energy(wf *gwf) { return grid_wf_energy(gwf, NULL); }
+
+/* ---- Gross-Pitaevskii --------------------------------------------------------------------- */
+
+static double gpe_energy_from_parts(wf *gwf, const double *parts, double g_nl) {
+  cgrid *g = gwf->grid;
+  const double dv = lg_step_factor(g->nx, g->ny, g->nz, g->step);
+  const double t = HBAR * HBAR / (2.0 * gwf->mass) * parts[HGRID_RED_KIN] * g->fft_norm2;
+  return t + parts[HGRID_RED_VEXT] * dv + 0.5 * g_nl * parts[HGRID_RED_QUART] * dv;
+}
+
+static double g_last_g = 0.0;
+
+void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize) {
+  cgrid *g = gwf->grid;
+  if (gwf->propagator != WF_2ND_ORDER_FFT) lg_fatal("grid_wf_gpe_propagate needs WF_2ND_ORDER_FFT");
+  if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
+  if (vext->nx != g->nx || vext->ny != g->ny || vext->nz != g->nz) lg_fatal("grid_wf_gpe_propagate: size mismatch");
+  check_fft_grid(g);
+  lg_shape s;
+  lg_squeeze(g->nx, g->ny, g->nz, &s);
+  if (s.m < 2) lg_fatal("grid_wf_gpe_propagate needs at least two non-trivial axes");
+  const REAL kx0 = g->kx0, ky0 = g->ky0, kz0 = g->kz0;
+  step_args sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.gpe = 1;
+  sa.vext = lg_r_in(vext);
+  sa.g = g_nl;
+  sa.c_re = cimag(time) / HBAR;
+  sa.c_im = -creal(time) / HBAR;
+  sa.cons = -(HBAR * HBAR / (2.0 * gwf->mass)) * (kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
+  const REAL complex tm = -I * (0.5 * time) * HBAR / (gwf->mass * 2.0);
+  sa.tm_re = creal(tm);
+  sa.tm_im = cimag(tm);
+  sa.want_energy = 1;
+  sa.want_post_norm = normalize ? 1 : 0;
+  fused_step(gwf, &sa);
+  grid_dev *d = g->dev;
+  if (normalize) {
+    lg_hip(hgrid_scalar_renorm(d->dscal + GS_SCALE, d->dscal + GS_RED + HGRID_RED_POST, gwf->norm,
+                               lg_step_factor(g->nx, g->ny, g->nz, g->step)),
+           "scalar_renorm");
+    d->pending = 1;
+  }
+  d->have_gpe_energy = 1;
+  g_last_g = g_nl;
+}
+
+REAL grid_wf_gpe_last_energy(wf *gwf) {
+  grid_dev *d = gwf->grid->dev;
+  if (!d->have_gpe_energy) lg_fatal("grid_wf_gpe_last_energy: no GPE step has run on this wavefunction");
+  double parts[HGRID_RED_NCHAN];
+  lg_hip(hgrid_memcpy_d2h(parts, d->dscal + GS_RED, sizeof(parts)), "hipMemcpy D2H (energies)");
+  return gpe_energy_from_parts(gwf, parts, g_last_g);
+}
+
+REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g_nl) {
+  cgrid *g = gwf->grid;
+  const double dv = lg_step_factor(g->nx, g->ny, g->nz, g->step);
+  double t = grid_wf_kinetic_energy_fft(gwf);
+  double v = grid_grid_expectation_value(g, vext);
+  /* (g/2) int |psi|^4 */
+  grid_dev *d = g->dev;
+  const double *scale;
+  const void *psi = lg_c_in(g, &scale);
+  if (!d->rho) lg_hip(hgrid_malloc((void **)&d->rho, d->n * sizeof(double)), "hipMalloc (density)");
+  lg_hip(hgrid_density(d->rho, psi, d->prec, (long)d->n, scale), "density");
+  double *scr = lg_partials(hgrid_reduce_scratch((long)d->n));
+  lg_hip(hgrid_sum_real(d->dscal + GS_TMP, d->rho, (long)d->n, 1, scr), "sum_real");
+  double q = fetch(d->dscal + GS_TMP);
+  return t + v + 0.5 * g_nl * q * dv;
+}
+
+INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy) {
+  grid_dev *d = gwf->grid->dev;
+  const INT batch = 16;
+  double *hist = NULL;
+  lg_ensure_device();
+  lg_hip(hgrid_malloc((void **)&hist, sizeof(double) * HGRID_RED_NCHAN * batch), "hipMalloc (energy history)");
+  double host[HGRID_RED_NCHAN * 16];
+  double prev = NAN, e = NAN;
+  INT done = -1, n = 0;
+  while (n < max_steps && done < 0) {
+    INT nb = (max_steps - n) < batch ? (max_steps - n) : batch;
+    for (INT b = 0; b < nb; ++b) {
+      grid_wf_gpe_propagate(gwf, vext, g_nl, -I * dt, 1);
+      lg_hip(hgrid_memcpy_d2d(hist + HGRID_RED_NCHAN * b, d->dscal + GS_RED, sizeof(double) * HGRID_RED_NCHAN),
+             "hipMemcpy D2D (energy history)");
+    }
+    lg_hip(hgrid_memcpy_d2h(host, hist, sizeof(double) * HGRID_RED_NCHAN * nb), "hipMemcpy D2H (energy history)");
+    for (INT b = 0; b < nb; ++b) {
+      e = gpe_energy_from_parts(gwf, host + HGRID_RED_NCHAN * b, g_nl);
+      /* host[b] is the energy of the state that entered step n + b + 1 */
+      if (!isnan(prev) && fabs(e - prev) < tol) {
+        done = n + b + 1;
+        break;
+      }
+      prev = e;
+    }
+    n += nb;
+  }
+  lg_hip(hgrid_free(hist), "hipFree");
+  if (energy) *energy = e;
+  return done < 0 ? n : done;
+}

@@ -1,0 +1,83 @@
+"""Seeded synthetic inputs for the split-operator workloads (numpy only).
+
+Coordinates follow the reference's grid mapping x = (i - nx/2) * step - x0 (src/cgrid.c:793-815,
+src/rgrid.c:806).  Every drawn parameter is returned in a dict so reports can print it.
+
+Workloads:
+  wavepacket_harmonic  128^3 complex128, step 0.2, mass 1: Gaussian packet of width 1.5 bohr
+                       (functional form of examples/wavepacket.c:159-180) in V = 0.5*0.01*r^2,
+                       centre U(-2,2)^3, momentum U(-1,1)^3; 100 real-time steps of dt = 0.05.
+  gpe_barrier          256^3 complex128, step 0.25: V = 0.5*0.01*r^2 + 0.05*exp(-r^2/2^2), g = 0.1,
+                       Gaussian (sigma = 1/sqrt(m*omega)) times exp(i theta), theta ~ U[0, 2pi) per
+                       point then a 3x3x3 periodic box average; imaginary-time relaxation.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def axes(n: int, step: float, origin: float = 0.0) -> np.ndarray:
+    return (np.arange(n) - n // 2) * step - origin
+
+
+def coords(shape, step):
+    nx, ny, nz = shape
+    return np.meshgrid(axes(nx, step), axes(ny, step), axes(nz, step), indexing="ij", sparse=True)
+
+
+def normalize(psi: np.ndarray, step: float) -> np.ndarray:
+    dv = step ** sum(1 for n in psi.shape if n > 1)
+    return psi / np.sqrt(np.sum(np.abs(psi) ** 2) * dv)
+
+
+def gaussian_packet(shape, step, width, centre, momentum) -> np.ndarray:
+    """exp(-|r - rc|^2 / w^2 + i p.(r - rc)), the form of examples/wavepacket.c:173-180."""
+    x, y, z = coords(shape, step)
+    dx, dy, dz = x - centre[0], y - centre[1], z - centre[2]
+    arg = -(dx * dx + dy * dy + dz * dz) / width**2 + 1j * (momentum[0] * dx + momentum[1] * dy + momentum[2] * dz)
+    return np.exp(arg).astype(np.complex128)
+
+
+def harmonic(shape, step, k=0.01) -> np.ndarray:
+    x, y, z = coords(shape, step)
+    return (0.5 * k * (x * x + y * y + z * z)).astype(np.float64) * np.ones(shape)
+
+
+def wavepacket_harmonic(seed: int = 0, n: int = 128, step: float = 0.2):
+    rng = np.random.default_rng(seed)
+    centre = rng.uniform(-2.0, 2.0, 3)
+    momentum = rng.uniform(-1.0, 1.0, 3)
+    shape = (n, n, n)
+    psi = normalize(gaussian_packet(shape, step, 1.5, centre, momentum), step)
+    v = harmonic(shape, step)
+    params = dict(shape=shape, step=step, mass=1.0, width=1.5, centre=centre.tolist(), momentum=momentum.tolist(),
+                  dt=0.05, steps=100, k=0.01)
+    return psi, v, params
+
+
+def box_average3(theta: np.ndarray) -> np.ndarray:
+    """3x3x3 periodic box average (the smoothing of the random phase field)."""
+    out = np.zeros_like(theta)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                out += np.roll(theta, (dx, dy, dz), axis=(0, 1, 2))
+    return out / 27.0
+
+
+def gpe_barrier(seed: int = 0, n: int = 256, step: float = 0.25, omega: float = 0.1, barrier: float = 0.05,
+                barrier_width: float = 2.0, g: float = 0.1):
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    x, y, z = coords(shape, step)
+    r2 = x * x + y * y + z * z
+    vext = (0.5 * omega**2 * r2 + barrier * np.exp(-r2 / barrier_width**2)).astype(np.float64)
+    vext = np.broadcast_to(vext, shape).copy()
+    theta = rng.uniform(0.0, 2.0 * np.pi, shape)
+    theta = box_average3(theta)
+    sigma2 = 1.0 / omega  # 1 / (m omega), m = 1
+    psi = np.exp(-r2 / (2.0 * sigma2)) * np.exp(1j * theta)
+    psi = normalize(psi.astype(np.complex128), step)
+    params = dict(shape=shape, step=step, mass=1.0, g=g, omega=omega, barrier=barrier, barrier_width=barrier_width,
+                  dt=0.05, tol=1e-10)
+    return psi, vext, params

@@ -1,0 +1,188 @@
+"""ORACLE — test infrastructure only.
+
+ctypes binding of ``oracle/liboracle.so`` (the C/OpenMP restatement of the reference hot
+path in ``oracle/oracle.c``).  Imported by tests/, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of ``bench.py`` — as the checker or the CPU baseline, never as the thing
+measured or shipped.  Arrays are numpy complex128 / float64 in row-major [nx][ny][nz].
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+P = C.c_void_p
+D = C.c_double
+L = C.c_long
+I = C.c_int
+
+_PROTOS = {
+    "or_fft3d": (None, [P, L, L, L, I]),
+    "or_cgrid_fft": (None, [P, L, L, L]),
+    "or_cgrid_inverse_fft": (None, [P, L, L, L]),
+    "or_cgrid_inverse_fft_norm": (None, [P, L, L, L]),
+    "or_cgrid_multiply": (None, [P, L, D, D]),
+    "or_integral_of_square": (D, [P, L, L, L, D]),
+    "or_integral_of_conjugate_product": (None, [P, P, L, L, L, D, P]),
+    "or_fft_laplace_expectation_value": (D, [P, L, L, L, D]),
+    "or_grid_expectation_value": (D, [P, P, L, L, L, D]),
+    "or_propagate_kinetic_fft": (None, [P, L, L, L, D, D, D, D, D, D, D]),
+    "or_propagate_potential": (None, [P, P, L, D, D, D]),
+    "or_wf_propagate": (None, [P, P, L, L, L, D, D, D, D, D, D, D]),
+    "or_wf_normalize": (D, [P, L, L, L, D, D]),
+    "or_kinetic_energy_fft": (D, [P, L, L, L, D, D]),
+    "or_energy_fft": (D, [P, P, L, L, L, D, D]),
+    "or_kinetic_energy_fd_periodic": (D, [P, L, L, L, D, D]),
+    "or_gpe_step": (None, [P, P, D, L, L, L, D, D, D, D, D, D, D, I, D]),
+    "or_gpe_energy": (D, [P, P, D, L, L, L, D, D]),
+    "or_num_threads": (I, []),
+    "or_set_num_threads": (None, [I]),
+}
+
+
+def build() -> None:
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout[-3000:] + out.stderr[-3000:])
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _c(a: np.ndarray) -> np.ndarray:
+    if a.dtype != np.complex128 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("oracle arrays must be C-contiguous complex128")
+    return a
+
+
+def _r(a: np.ndarray) -> np.ndarray:
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("oracle arrays must be C-contiguous float64")
+    return a
+
+
+def fft3d(a: np.ndarray, sign: int) -> np.ndarray:
+    """Unnormalised 3-D DFT of a copy (sign -1 forward, +1 backward)."""
+    b = _c(np.array(a, dtype=np.complex128, order="C", copy=True))
+    nx, ny, nz = b.shape
+    lib().or_fft3d(b.ctypes.data, nx, ny, nz, sign)
+    return b
+
+
+def inverse_fft_norm(a: np.ndarray) -> np.ndarray:
+    b = _c(np.array(a, dtype=np.complex128, order="C", copy=True))
+    lib().or_cgrid_inverse_fft_norm(b.ctypes.data, *b.shape)
+    return b
+
+
+def integral_of_square(a, step) -> float:
+    a = _c(a)
+    return lib().or_integral_of_square(a.ctypes.data, *a.shape, step)
+
+
+def integral_of_conjugate_product(a, b, step) -> complex:
+    a, b = _c(a), _c(b)
+    out = np.zeros(2)
+    lib().or_integral_of_conjugate_product(a.ctypes.data, b.ctypes.data, *a.shape, step, out.ctypes.data)
+    return complex(out[0], out[1])
+
+
+def laplace_expectation(ak, step) -> float:
+    ak = _c(ak)
+    return lib().or_fft_laplace_expectation_value(ak.ctypes.data, *ak.shape, step)
+
+
+def grid_expectation(a, v, step) -> float:
+    a, v = _c(a), _r(v)
+    return lib().or_grid_expectation_value(a.ctypes.data, v.ctypes.data, *a.shape, step)
+
+
+def propagate_kinetic_fft(ak, step, mass, k0, time) -> np.ndarray:
+    b = _c(np.array(ak, dtype=np.complex128, order="C", copy=True))
+    t = complex(time)
+    lib().or_propagate_kinetic_fft(b.ctypes.data, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag)
+    return b
+
+
+def propagate_potential(a, pot, tstep, cons) -> np.ndarray:
+    b = _c(np.array(a, dtype=np.complex128, order="C", copy=True))
+    pot = _c(np.ascontiguousarray(pot, dtype=np.complex128))
+    t = complex(tstep)
+    lib().or_propagate_potential(b.ctypes.data, pot.ctypes.data, b.size, t.real, t.imag, cons)
+    return b
+
+
+def wf_propagate(a, pot, step, mass, k0, time) -> np.ndarray:
+    """One reference grid_wf_propagate (WF_2ND_ORDER_FFT) step on a copy; pot may be None."""
+    b = _c(np.array(a, dtype=np.complex128, order="C", copy=True))
+    t = complex(time)
+    pp = None
+    if pot is not None:
+        pot = _c(np.ascontiguousarray(pot, dtype=np.complex128))
+        pp = pot.ctypes.data
+    lib().or_wf_propagate(b.ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag)
+    return b
+
+
+def wf_propagate_inplace(b: np.ndarray, pot, step, mass, k0, time) -> None:
+    t = complex(time)
+    pp = None if pot is None else _c(pot).ctypes.data
+    lib().or_wf_propagate(_c(b).ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag)
+
+
+def wf_normalize(a, step, target=1.0):
+    b = _c(np.array(a, dtype=np.complex128, order="C", copy=True))
+    n = lib().or_wf_normalize(b.ctypes.data, *b.shape, step, target)
+    return b, n
+
+
+def kinetic_energy_fft(a, step, mass) -> float:
+    a = _c(a)
+    return lib().or_kinetic_energy_fft(a.ctypes.data, *a.shape, step, mass)
+
+
+def energy_fft(a, v, step, mass) -> float:
+    a = _c(a)
+    vp = None if v is None else _r(v).ctypes.data
+    return lib().or_energy_fft(a.ctypes.data, vp, *a.shape, step, mass)
+
+
+def kinetic_energy_fd_periodic(a, step, mass) -> float:
+    a = _c(a)
+    return lib().or_kinetic_energy_fd_periodic(a.ctypes.data, *a.shape, step, mass)
+
+
+def gpe_step_inplace(b, vext, g, step, mass, k0, time, normalize=True, target=1.0) -> None:
+    t = complex(time)
+    lib().or_gpe_step(_c(b).ctypes.data, _r(vext).ctypes.data, g, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real,
+                      t.imag, 1 if normalize else 0, target)
+
+
+def gpe_energy(a, vext, g, step, mass) -> float:
+    a = _c(a)
+    return lib().or_gpe_energy(a.ctypes.data, _r(vext).ctypes.data, g, *a.shape, step, mass)
+
+
+def num_threads() -> int:
+    return lib().or_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    lib().or_set_num_threads(n)

@@ -1,0 +1,92 @@
+"""CPU checks of the C-ABI: libgrid.so loads, exports every function declared in include/*.h,
+and the ctypes mirrors of the public structs have the C layout.  No GPU call is made."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+import libgrid_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", "grid.h"), os.path.join(ROOT, "include", "grid_hip.h")]
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//.*", "", text)
+    names = set()
+    for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_ \*]*?)\b([a-z][a-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
+        name = m.group(2)
+        if name in ("if", "while", "sizeof", "return", "complex"):
+            continue
+        names.add(name)
+    return names
+
+
+def test_library_loads_and_exports_all_declared_symbols():
+    lib = libgrid_amd.load()
+    missing = []
+    total = 0
+    for h in HEADERS:
+        for name in sorted(declared_functions(h)):
+            total += 1
+            if not hasattr(lib, name):
+                missing.append(name)
+    assert total > 80
+    assert not missing, missing
+
+
+def test_prototype_table_covers_only_exported_names():
+    lib = libgrid_amd.load()
+    for name in libgrid_amd.PROTOTYPES:
+        assert hasattr(lib, name), name
+
+
+def test_struct_layout_matches_c():
+    src = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "grid.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(cgrid), offsetof(cgrid, fft_norm), offsetof(cgrid, dev),
+         sizeof(rgrid), offsetof(rgrid, nz2), sizeof(wf));
+  printf("%zu %zu %zu\n", offsetof(wf, mass), offsetof(wf, ts_func), offsetof(wf, kmax));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=gnu11", "-I" + os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    vals = list(map(int, out))
+    assert vals[0] == C.sizeof(libgrid_amd.cgrid)
+    assert vals[1] == libgrid_amd.cgrid.fft_norm.offset
+    assert vals[2] == libgrid_amd.cgrid.dev.offset
+    assert vals[3] == C.sizeof(libgrid_amd.rgrid)
+    assert vals[4] == libgrid_amd.rgrid.nz2.offset
+    assert vals[5] == C.sizeof(libgrid_amd.wf)
+    assert vals[6] == libgrid_amd.wf.mass.offset
+    assert vals[7] == libgrid_amd.wf.ts_func.offset
+    assert vals[8] == libgrid_amd.wf.kmax.offset
+
+
+def test_no_gpu_means_loud_failure():
+    """Without a HIP device every computational call aborts with a libgrid message (no CPU path)."""
+    import torch  # noqa: F401  (only to know whether a GPU exists here)
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    code = (
+        "import libgrid_amd, ctypes; lib = libgrid_amd.load();"
+        "g = lib.cgrid_alloc(4, 4, 4, 0.1, None, None, b'x'); lib.cgrid_fft(g)"
+    )
+    p = subprocess.run(["python", "-c", code], capture_output=True, text=True, cwd=ROOT,
+                       env=dict(os.environ, HIP_VISIBLE_DEVICES="", PYTHONPATH=ROOT))
+    assert p.returncode != 0
+    assert "libgrid:" in p.stderr

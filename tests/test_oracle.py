@@ -1,0 +1,265 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+The reference ships no tests or golden vectors for this path (its examples only print values),
+so each oracle function is checked against something it does not contain:
+  * FFT: numpy.fft (pocketfft) golden vectors committed in tests/golden, brute-force O(N^2)
+    DFTs, Parseval / round trip / shift theorem / plane waves, and the cosine round trip of
+    the reference's examples/cfft.c;
+  * kinetic phase: closed-form free-particle evolution of a Gaussian packet (also in a moving
+    frame, k0 != 0) and the reference's Nyquist convention;
+  * potential phase and the full step: constant potential = global phase, absorbing
+    potential = decay, Ehrenfest dynamics in a harmonic trap, time reversibility, norm
+    conservation, the harmonic-oscillator ground state E0 = 3/2 hbar omega;
+  * norm and energies: analytic Gaussian integrals; FD energy converges as O(h^2);
+  * GPE composition: uniform condensates (closed form) and the g = 0 limit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from libgrid_amd import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fft_golden.npz")
+
+
+def brute_dft(a, sign):
+    out = a.astype(np.complex128)
+    for axis, n in enumerate(a.shape):
+        k = np.arange(n)
+        m = np.exp(sign * 2j * np.pi * np.outer(k, k) / n)
+        out = np.moveaxis(np.tensordot(m, np.moveaxis(out, axis, 0), axes=(1, 0)), 0, axis)
+    return out
+
+
+def test_fft_matches_numpy_golden(orc):
+    g = np.load(GOLDEN)
+    tags = sorted(k[3:] for k in g.files if k.startswith("in_"))
+    assert len(tags) >= 6
+    for tag in tags:
+        a = g["in_" + tag]
+        fwd = orc.fft3d(a, -1)
+        bwd = orc.fft3d(a, +1)
+        assert rel_l2(fwd, g["fwd_" + tag]) < 1e-14, tag
+        assert rel_l2(bwd, g["bwd_" + tag]) < 1e-14, tag
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (2, 3, 5), (1, 1, 7), (3, 1, 8), (1, 6, 1)])
+def test_fft_brute_force(orc, shape):
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    assert rel_l2(orc.fft3d(a, -1), brute_dft(a, -1)) < 1e-14
+    assert rel_l2(orc.fft3d(a, +1), brute_dft(a, +1)) < 1e-14
+
+
+def test_fft_invariants(orc):
+    rng = np.random.default_rng(3)
+    shape = (8, 16, 32)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    b = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    n = a.size
+    fa = orc.fft3d(a, -1)
+    # Parseval
+    assert abs(np.sum(np.abs(fa) ** 2) / (n * np.sum(np.abs(a) ** 2)) - 1) < 1e-14
+    # round trip through the normalised inverse
+    assert rel_l2(orc.inverse_fft_norm(fa), a) < 1e-14
+    # linearity
+    assert rel_l2(orc.fft3d(2.0 * a - 3j * b, -1), 2.0 * fa - 3j * orc.fft3d(b, -1)) < 1e-14
+    # shift theorem: a(x - 1) along z -> multiply by exp(-2 pi i k / nz)
+    sa = np.roll(a, 1, axis=2)
+    kz = np.arange(shape[2])
+    assert rel_l2(orc.fft3d(sa, -1), fa * np.exp(-2j * np.pi * kz / shape[2])[None, None, :]) < 1e-13
+    # a plane wave maps to N * delta at its wave vector
+    i, j, k = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    pw = np.exp(2j * np.pi * (3 * i / shape[0] + 5 * j / shape[1] + 7 * k / shape[2]))
+    fp = orc.fft3d(pw, -1)
+    expect = np.zeros(shape, complex)
+    expect[3, 5, 7] = n
+    assert np.max(np.abs(fp - expect)) < 1e-9
+
+
+def test_fft_cosine_round_trip_known_answer(orc):
+    """examples/cfft.c:32-65: cos standing waves -> FFT -> normalised inverse FFT = input."""
+    nx, ny, nz, step = 16, 32, 64, 0.5
+    x, y, z = W.coords((nx, ny, nz), step)
+    f = (np.cos(x * 2 * np.pi / (nx * step)) + np.cos(y * 2 * np.pi / (ny * step)) + np.cos(z * 2 * np.pi / (nz * step)))
+    f = np.broadcast_to(f, (nx, ny, nz)).astype(np.complex128).copy()
+    fk = orc.fft3d(f, -1)
+    n = f.size
+    # each cosine contributes N/2 at k = +-1 along its axis (with the sign of cos(-pi) = -1 at i = -n/2)
+    mask = np.abs(fk) > 1e-6 * n
+    assert mask.sum() == 6
+    assert abs(np.sum(np.abs(fk[mask])) - 3 * n) < 1e-8 * n
+    assert rel_l2(orc.inverse_fft_norm(fk), f) < 1e-14
+
+
+def free_gaussian_1d(x, a, p, t, mass=1.0, hbar=1.0):
+    """Free evolution of exp(-a x^2 + i p x): (1 + 2 i a hbar t / m)^(-1/2) exp((-a x^2 + i p x - i hbar p^2 t / 2m) / D)."""
+    d = 1.0 + 2j * a * hbar * t / mass
+    return d**-0.5 * np.exp((-a * x * x + 1j * p * x - 1j * hbar * p * p * t / (2 * mass)) / d)
+
+
+def test_free_particle_closed_form(orc):
+    """Kinetic phase and FFTs: V = 0 split-operator steps equal the exact free evolution."""
+    n, step, w, mass = 128, 0.2, 1.5, 1.3
+    p = np.array([0.7, -0.4, 0.25])
+    shape = (n, n, n)
+    a = 1.0 / w**2
+    x, y, z = W.coords(shape, step)
+    psi0 = (np.exp(-a * x * x + 1j * p[0] * x) * np.exp(-a * y * y + 1j * p[1] * y) * np.exp(-a * z * z + 1j * p[2] * z))
+    psi = np.ascontiguousarray(psi0, dtype=np.complex128)
+    dt, steps = 0.05, 10
+    for _ in range(steps):
+        orc.wf_propagate_inplace(psi, None, step, mass, (0.0, 0.0, 0.0), dt)
+    t = dt * steps
+    exact = free_gaussian_1d(x, a, p[0], t, mass) * free_gaussian_1d(y, a, p[1], t, mass) * free_gaussian_1d(z, a, p[2], t, mass)
+    assert rel_l2(psi, exact) < 1e-12
+
+
+def test_moving_frame_closed_form(orc):
+    """k0 != 0: the reference propagates with (k - k0)^2 and adds cons = -hbar^2 k0^2 / 2m to V.
+    With V = 0 the exact result is exp(-i cons t) exp(i k0 x) * free[p - k0]."""
+    n, step, w, mass = 128, 0.2, 1.5, 1.0
+    shape = (1, n, n)
+    a = 1.0 / w**2
+    l = 2 * np.pi / (n * step)
+    k0 = (0.0, 2 * l, -3 * l)  # multiples of 2 pi / L as the reference requires (src/cgrid.c:290-300)
+    p = (0.0, 0.5, 0.3)
+    _, y, z = W.coords(shape, step)
+    psi = np.ascontiguousarray(np.exp(-a * y * y + 1j * p[1] * y) * np.exp(-a * z * z + 1j * p[2] * z) * np.ones(shape))
+    zero = np.zeros(shape, complex)
+    dt, steps = 0.05, 10
+    for _ in range(steps):
+        orc.wf_propagate_inplace(psi, zero, step, mass, k0, dt)
+    t = dt * steps
+    cons = -(k0[1] ** 2 + k0[2] ** 2) / (2 * mass)
+    exact = (np.exp(-1j * cons * t) * np.exp(1j * (k0[1] * y + k0[2] * z)) * free_gaussian_1d(y, a, p[1] - k0[1], t, mass) *
+             free_gaussian_1d(z, a, p[2] - k0[2], t, mass))
+    assert rel_l2(psi, np.broadcast_to(exact, shape)) < 1e-12
+
+
+def test_kinetic_nyquist_convention(orc):
+    """Index n/2 is the positive frequency +n/2 (src/grid_wf_fft.c:173-187); visible when k0 != 0."""
+    n, step, mass = 16, 0.3, 1.0
+    k0 = (0.0, 0.0, 0.4)
+    ak = np.zeros((1, 1, n), complex)
+    ak[0, 0, n // 2] = 1.0
+    out = orc.propagate_kinetic_fft(ak, step, mass, k0, 0.1)
+    kn = (n // 2) * 2 * np.pi / (n * step) - k0[2]
+    assert abs(out[0, 0, n // 2] - np.exp(-1j * 0.1 * kn * kn / (2 * mass))) < 1e-14
+
+
+def test_constant_and_absorbing_potential(orc):
+    """A constant potential V0 is a global phase exp(-i V0 t); V0 - i G also decays as exp(-G t)."""
+    rng = np.random.default_rng(5)
+    shape = (8, 8, 16)
+    psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    v0, gam, dt = 0.37, 0.2, 0.05
+    pot = np.full(shape, v0 - 1j * gam)
+    ref_free = orc.wf_propagate(psi, None, 0.2, 1.0, (0, 0, 0), dt)
+    out = orc.wf_propagate(psi, pot, 0.2, 1.0, (0, 0, 0), dt)
+    assert rel_l2(out, ref_free * np.exp(-1j * v0 * dt) * np.exp(-gam * dt)) < 1e-13
+    # imaginary time: exp(-tau V0)
+    ref_free_it = orc.wf_propagate(psi, None, 0.2, 1.0, (0, 0, 0), -1j * dt)
+    out_it = orc.wf_propagate(psi, np.full(shape, v0 + 0j), 0.2, 1.0, (0, 0, 0), -1j * dt)
+    assert rel_l2(out_it, ref_free_it * np.exp(-dt * v0)) < 1e-13
+
+
+def test_norm_and_energies_analytic(orc):
+    n, step, w, mass = 128, 0.2, 1.5, 1.0
+    rc = np.array([0.6, -1.1, 0.3])
+    p = np.array([0.4, 0.9, -0.5])
+    shape = (n, n, n)
+    psi = W.gaussian_packet(shape, step, w, rc, p)
+    assert abs(orc.integral_of_square(psi, step) / (np.pi * w * w / 2) ** 1.5 - 1) < 1e-13
+    psi = W.normalize(psi, step)
+    t = orc.kinetic_energy_fft(psi, step, mass)
+    assert abs(t - (np.dot(p, p) + 3 / w**2) / (2 * mass)) < 1e-12
+    v = W.harmonic(shape, step)
+    ev = orc.grid_expectation(psi, v, step)
+    assert abs(ev - 0.5 * 0.01 * (3 * w * w / 4 + np.dot(rc, rc))) < 1e-12
+    assert abs(orc.energy_fft(psi, v, step, mass) - (t + ev)) < 1e-13
+    # the FFT laplace expectation value of the transformed state is -|k|^2 weighted, as in the reference
+    assert abs(-orc.laplace_expectation(orc.fft3d(psi, -1), step) / (2 * mass) - t) < 1e-12
+
+
+def test_fd_kinetic_energy_converges(orc):
+    """The 7-point FD energy (src/cgrid_diff.c:303-330) approaches the analytic value as O(h^2)."""
+    w, mass, p = 1.5, 1.0, np.array([0.3, 0.0, -0.2])
+    exact = (np.dot(p, p) + 3 / w**2) / (2 * mass)
+    errs = []
+    for n, step in ((64, 0.4), (128, 0.2)):
+        psi = W.normalize(W.gaussian_packet((n, n, n), step, w, (0, 0, 0), p), step)
+        errs.append(abs(orc.kinetic_energy_fd_periodic(psi, step, mass) - exact))
+    assert errs[1] < 1e-2 * exact
+    assert 3.0 < errs[0] / errs[1] < 5.0  # second order
+
+
+def test_ehrenfest_harmonic(orc):
+    """<x>(t) = x_c cos(wt) + p/(m w) sin(wt) for V = 1/2 m w^2 x^2, up to the O(dt^2) splitting error."""
+    n, step, mass, omega = 128, 0.4, 1.0, 0.1
+    shape = (1, 1, n)
+    x = W.axes(n, step)
+    xc, p = 1.2, 0.8
+    w2 = 2.0 / (mass * omega)  # coherent-state width: the packet does not breathe or reach the box edge
+    psi = np.ascontiguousarray(W.normalize(np.exp(-(x - xc) ** 2 / w2 + 1j * p * (x - xc)).reshape(shape), step))
+    v = (0.5 * mass * omega**2 * x * x).reshape(shape).astype(complex)
+    dt, steps = 0.05, 200
+    for _ in range(steps):
+        orc.wf_propagate_inplace(psi, v, step, mass, (0, 0, 0), dt)
+    t = dt * steps
+    xm = np.sum(np.abs(psi.ravel()) ** 2 * x) * step
+    assert abs(xm - (xc * np.cos(omega * t) + p / (mass * omega) * np.sin(omega * t))) < 1e-4
+
+
+def test_reversibility_and_norm(orc):
+    psi, v, prm = W.wavepacket_harmonic(seed=11, n=32, step=0.8)
+    pot = v.astype(complex)
+    step, mass, dt = prm["step"], prm["mass"], prm["dt"]
+    fwd = orc.wf_propagate(psi, pot, step, mass, (0, 0, 0), dt)
+    back = orc.wf_propagate(fwd, pot, step, mass, (0, 0, 0), -dt)
+    assert rel_l2(back, psi) < 1e-13
+    assert abs(orc.integral_of_square(fwd, step) - orc.integral_of_square(psi, step)) < 1e-13
+
+
+def test_imaginary_time_ground_state(orc):
+    """Imaginary-time relaxation in V = 1/2 (0.01) r^2 reaches E0 = 3/2 hbar omega = 0.15."""
+    n, step, mass = 32, 0.8, 1.0
+    shape = (n, n, n)
+    v = W.harmonic(shape, step)
+    psi = np.ascontiguousarray(W.normalize(W.gaussian_packet(shape, step, 3.0, (0.5, 0, 0), (0, 0, 0)), step))
+    pot = v.astype(complex)
+    for _ in range(300):
+        orc.wf_propagate_inplace(psi, pot, step, mass, (0, 0, 0), -1j * 0.5)
+        psi[...] = orc.wf_normalize(psi, step)[0]
+    assert abs(orc.energy_fft(psi, v, step, mass) - 0.15) < 2e-4
+
+
+def test_gpe_uniform_condensate(orc):
+    """|psi|^2 = rho0 everywhere: the g|psi|^2 term is the constant g rho0 (phase exp(-i g rho0 t));
+    E_GP = hbar^2 k^2 / 2m + (g/2) rho0^2 Vol for a plane wave of unit norm."""
+    n, step, mass, g = 16, 0.5, 1.0, 0.7
+    shape = (n, n, n)
+    vol = (n * step) ** 3
+    rho0 = 1.0 / vol
+    i, j, k = np.meshgrid(*[np.arange(n)] * 3, indexing="ij")
+    kv = 2 * np.pi / (n * step)
+    psi = np.sqrt(rho0) * np.exp(2j * np.pi * (2 * k) / n)  # k_z = 2 kv
+    psi = np.ascontiguousarray(psi.astype(np.complex128))
+    vext = np.zeros(shape)
+    e = orc.gpe_energy(psi, vext, g, step, mass)
+    assert abs(e - ((2 * kv) ** 2 / (2 * mass) + 0.5 * g * rho0)) < 1e-12
+    b = psi.copy()
+    dt = 0.1
+    orc.gpe_step_inplace(b, vext, g, step, mass, (0, 0, 0), dt, normalize=False)
+    expect = psi * np.exp(-1j * dt * ((2 * kv) ** 2 / (2 * mass) + g * rho0))
+    assert rel_l2(b, expect) < 1e-13
+
+
+def test_gpe_g0_equals_plain_step(orc):
+    psi, v, prm = W.wavepacket_harmonic(seed=2, n=32, step=0.8)
+    a = psi.copy()
+    orc.gpe_step_inplace(a, v, 0.0, prm["step"], 1.0, (0, 0, 0), -0.05j, normalize=True)
+    b, _ = orc.wf_normalize(orc.wf_propagate(psi, v.astype(complex), prm["step"], 1.0, (0, 0, 0), -0.05j), prm["step"])
+    assert rel_l2(a, b) < 1e-15
