@@ -82,15 +82,26 @@ def pass_timings(lib, LG, psi_dev, n, reps=10):
 
     kinds = {
         # name: (pass, launches per step, algorithmic bytes per point per launch)
-        "z_fwd_pre": (mk(2, flags=LG.PF_PRESCALE | LG.PF_PRE_RHO | LG.PF_PRE_RED, scale=scale, rho_out=rho, vext=vext), 1, 48),
+        "z_fwd_pre": (mk(2, flags=LG.PF_PRESCALE | LG.PF_PRE_VEFF | LG.PF_PRE_RED, scale=scale, rho_out=rho, vext=vext,
+                         g=0.1), 1, 48),
         "y_fft": (mk(1), 4, 32),
         "x_kin": (mk(0, nfft=2, flags=LG.PF_MID_KIN | LG.PF_MID_KRED, ktab_a=kt, ktab_1=kt, ktab_2=kt, k2_a=k2, k2_1=k2,
                      k2_2=k2), 2, 32),
-        "z_gpe": (mk(2, nfft=2, dir1=+1, dir2=-1, flags=LG.PF_MID_GPE, vext=vext, rho=rho, g=0.1, c_re=-0.05, c_im=0.0), 1, 48),
+        "z_gpe": (mk(2, nfft=2, dir1=+1, dir2=-1, flags=LG.PF_MID_GPE, rho=rho, c_re=-0.05, c_im=0.0), 1, 40),
         "z_inv_post": (mk(2, dir1=+1, flags=LG.PF_POST_RED), 1, 32),
     }
     t = LG.HipTimer()
     out = {}
+    # measured ceiling: a streaming copy of one grid (N * 16 B read + N * 16 B written)
+    for _ in range(2):
+        lib.hgrid_stream_copy(work, psi_dev, N * 16)
+    t.start()
+    for _ in range(reps):
+        lib.hgrid_stream_copy(work, psi_dev, N * 16)
+    ms = t.stop() / reps
+    out["_copy"] = dict(ms=ms, count=0, bytes=32 * N, gbps=32 * N / (ms * 1e-3) / 1e9)
+    log(f"stream copy    {ms * 1e3:8.1f} us  {out['_copy']['gbps']:7.0f} GB/s  (measured ceiling)")
+    lib.hgrid_memcpy_d2d(work, psi_dev, N * 16)
     for name, (p, count, bpp) in kinds.items():
         for _ in range(2):
             if lib.hgrid_fft_pass(C.byref(p)):
@@ -106,6 +117,24 @@ def pass_timings(lib, LG, psi_dev, n, reps=10):
     for p in (work, rho, vext, kt, k2, tw, scale, part):
         lib.hgrid_free(p)
     return out
+
+
+# template instantiation of each pass kind in the rocprofv3 summaries (precision, length, layout, kind)
+PASS_SYMBOL = {"y_fft": "256, false, 0>", "x_kin": "256, false, 1>", "z_gpe": "256, true, 3>"}
+
+
+def profiled_traffic(pass_name, n):
+    """HBM bytes per launch of a pass from the committed PMC summary (FETCH_SIZE + WRITE_SIZE,
+    calibrated against the streaming copy), or None when it is not available for this pass."""
+    path = os.path.join(ROOT, "profiles", "gpe256_fft_pass_hbm_traffic.txt")
+    key = PASS_SYMBOL.get(pass_name)
+    if n != 256 or key is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if "fft_pass_kernel<HIP_vector_type<double, 2u>, " + key in line:
+            parts = line.split()
+            return (float(parts[0]) + float(parts[1])) * 1024 * 1024
+    return None
 
 
 def cpu_baseline(psi, vext, prm, steps):
@@ -127,13 +156,67 @@ def cpu_baseline(psi, vext, prm, steps):
                        f"{el:.1f} s")
 
 
+def summarize_rocpd(stats_dir, fetch_dir, write_dir, command):
+    """Text summary of rocprofv3 runs of this benchmark: kernel time statistics, and HBM bytes per
+    dispatch from FETCH_SIZE / WRITE_SIZE scaled by what the same counters read for the streaming
+    copy kernel (which moves a known number of bytes: one grid read and one grid written)."""
+    import glob
+    import sqlite3
+
+    def db(d):
+        f = sorted(glob.glob(os.path.join(d, "**", "*.db"), recursive=True))
+        if not f:
+            raise FileNotFoundError(f"no rocpd database under {d}")
+        return sqlite3.connect(f[0])
+
+    out = [f"command: {command}", ""]
+    if stats_dir:
+        c = db(stats_dir)
+        out.append("kernel statistics (rocprofv3 --kernel-trace --stats): calls, total us, average us, % of GPU time")
+        for name, calls, tot, avg, pct in c.execute("select name, total_calls, total_duration, average, percentage from top_kernels"):
+            out.append(f"  {calls:5d} {tot:12.1f} {avg:10.2f} {pct:6.2f}%  {name}")
+        out.append("")
+    if fetch_dir and write_dir:
+        vals = {}
+        for tag, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+            c = db(d)
+            for name, v, n in c.execute("select kernel_name, avg(value), count(*) from counters_collection where counter_name = ? "
+                                        "group by kernel_name", (tag,)):
+                vals.setdefault(name, {})[tag] = (v, n)
+        copy = [k for k in vals if "k_stream_copy" in k]
+        if copy:
+            cf, cw = vals[copy[0]]["FETCH_SIZE"][0], vals[copy[0]]["WRITE_SIZE"][0]
+            grid_kib = 256.0 ** 3 * 16 / 1024.0
+            ff, fw = grid_kib / cf, grid_kib / cw
+            out.append(f"calibration: stream copy of one 256^3 complex128 grid ({grid_kib:.0f} KiB each way) reads "
+                       f"FETCH_SIZE {cf:.0f} KiB, WRITE_SIZE {cw:.0f} KiB -> factors fetch x{ff:.3f}, write x{fw:.3f}")
+            out.append("HBM bytes per dispatch (calibrated): MiB read, MiB written, dispatches  kernel")
+            for name in sorted(vals):
+                if "FETCH_SIZE" not in vals[name] or "WRITE_SIZE" not in vals[name]:
+                    continue
+                r = vals[name]["FETCH_SIZE"][0] * ff / 1024.0
+                w = vals[name]["WRITE_SIZE"][0] * fw / 1024.0
+                out.append(f"  {r:10.1f} {w:10.1f} {vals[name]['FETCH_SIZE'][1]:5d}  {name}")
+    return "\n".join(out)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-rocpd":
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--summarize-rocpd", action="store_true")
+        ap.add_argument("--stats", default=None)
+        ap.add_argument("--fetch", default=None)
+        ap.add_argument("--write", default=None)
+        ap.add_argument("--command", default="")
+        a = ap.parse_args()
+        print(summarize_rocpd(a.stats, a.fetch, a.write, a.command))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -178,12 +261,15 @@ def main():
 
     psi_dev = C.cast(w.contents.grid.contents.dev, C.POINTER(_dev)).contents.d
     kinds = pass_timings(lib, LG, C.c_void_p(psi_dev), n)
+    copy = kinds.pop("_copy")
     dom = max(kinds.items(), key=lambda kv: kv[1]["ms"] * kv[1]["count"])
     dname, d = dom
     sum_ms = sum(v["ms"] * v["count"] for v in kinds.values())
     N = n**3
     model_bytes = 400 * N  # (4 FFTs x 3 sweeps x 2 x 16 B + 16 B potential) per point
     actual_bytes = sum(v["bytes"] * v["count"] for v in kinds.values())
+
+    traffic = profiled_traffic(dname, n)
 
     cpu = None
     if not args.no_cpu:
@@ -208,7 +294,7 @@ def main():
                                f"renormalised, start-of-step energy)", "grid": [n, n, n], "step_bohr": step,
                    "passes_per_step": 9},
         "roofline": {"bound": "hbm", "kernel": f"fft_pass {dname}", "achieved": d["gbps"], "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": d["gbps"] / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": d["gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]},
         "step_roofline": {"model_bytes_per_step": model_bytes, "achieved_GBps": model_bytes / (ms_step * 1e-3) / 1e9,
                           "frac": model_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
@@ -216,6 +302,7 @@ def main():
                           "actual_GBps": actual_bytes / (ms_step * 1e-3) / 1e9,
                           "sum_of_pass_ms": sum_ms},
         "pass_ms": {k: v["ms"] for k, v in kinds.items()},
+        "copy_ceiling_GBps": copy["gbps"],
         "energy": energy,
         "cpu_baseline": cpu,
     }

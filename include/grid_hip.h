@@ -44,6 +44,9 @@ int hgrid_event_destroy(void *ev);
 int hgrid_event_record(void *ev);
 int hgrid_event_elapsed_ms(void *start, void *stop, float *ms);   /* synchronizes on stop */
 int hgrid_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* Streaming copy of `bytes` (a multiple of 16) with 16-byte loads/stores: the measured HBM
+ * ceiling the FFT passes are compared with, and the calibration kernel for PMC byte counts. */
+int hgrid_stream_copy(void *dst, const void *src, size_t bytes);
 
 /* ---- storage conversion ----------------------------------------------------------------- */
 
@@ -69,9 +72,11 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
 #define HGRID_PF_MID_KIN    0x0008u  /* mid: psi *= ktab_a[n] * ktab_1[c1] * ktab_2[c2] */
 #define HGRID_PF_MID_KRED   0x0010u  /* mid: partial sum of k^2 |psi_k|^2 (before MID_KIN) */
 #define HGRID_PF_MID_POT    0x0020u  /* mid: psi *= mid * exp(c * (cons + pot)), complex pot */
-#define HGRID_PF_MID_GPE    0x0040u  /* mid: psi *= mid * exp(c * (cons + vext + g * rho)) */
+#define HGRID_PF_MID_GPE    0x0040u  /* mid: psi *= mid * exp(c * (cons + vext + g * rho)), or
+                                        * exp(c * (cons + rho)) when vext is NULL (rho holds V_eff) */
 #define HGRID_PF_POST_RED   0x0080u  /* post: partial sum of |psi|^2 of the stored values */
 #define HGRID_PF_NOSTORE    0x0100u  /* reduce only; do not write the array */
+#define HGRID_PF_PRE_VEFF   0x0200u  /* pre: rho_out = vext + g |psi|^2 (GPE potential) instead of |psi|^2 */
 
 /* Reduction channels (partials[chan * nblocks + block]). */
 #define HGRID_RED_NORM   0   /* sum |psi|^2 of the pass input   (PRE_RED) */
@@ -93,7 +98,7 @@ typedef struct hgrid_pass {
   const void *tw;        /* W_L^m = exp(-2 pi i m / L), m < L, precision prec, L = length of axis */
   const double *scale;   /* PRESCALE: device complex scalar {re, im} */
   double post;           /* factor applied to the stored values (1.0 = none) */
-  double mid;            /* factor applied at MID_POT / MID_GPE / MID_SCALE */
+  double mid;            /* factor applied at MID_POT / MID_GPE */
   /* kinetic tables (precision prec): factor for the element index along axis, and for the two
    * other squeezed coordinates c1, c2 of the pencil (see hgrid_pass_coords) */
   const void *ktab_a, *ktab_1, *ktab_2;
@@ -121,6 +126,26 @@ int hgrid_fft_supported(long n);
 /* out[c] = sum_{b < nblocks} partials[c * nblocks + b] for c < nchan, deterministic order,
  * FP64 Neumaier-compensated.  out is a device array. */
 int hgrid_reduce_partials(double *out, const double *partials, long nblocks, int nchan);
+
+/* Up to HGRID_MAX_JOBS reductions of pass partials in one launch (the end of a fused step):
+ * job j sums channels [chan_first, chan_first + nchan) of its partials into out[0..nchan).
+ * Then, if scale != NULL: with target > 0, *scale = {sqrt(target / (*norm_sum * dv)), 0}
+ * (renormalisation); with target <= 0, *scale = {1, 0} (a consumed pending scale). */
+#define HGRID_MAX_JOBS 4
+typedef struct hgrid_reduce_job {
+  const double *partials;
+  long nblocks;
+  int chan_first, nchan;
+  double *out;
+} hgrid_reduce_job;
+typedef struct hgrid_finalize {
+  hgrid_reduce_job job[HGRID_MAX_JOBS];
+  int njobs;
+  double *scale;
+  const double *norm_sum;
+  double target, dv;
+} hgrid_finalize;
+int hgrid_step_finalize(const hgrid_finalize *f);
 
 /* Elementwise reductions over device arrays (n elements), written to the device scalar *out
  * (deterministic; scratch must hold hgrid_reduce_scratch(n) doubles). */

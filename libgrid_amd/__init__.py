@@ -176,6 +176,7 @@ PF_MID_POT = 0x0020
 PF_MID_GPE = 0x0040
 PF_POST_RED = 0x0080
 PF_NOSTORE = 0x0100
+PF_PRE_VEFF = 0x0200
 HGRID_C128 = 0
 HGRID_C64 = 1
 
@@ -278,6 +279,7 @@ PROTOTYPES = {
     "hgrid_memcpy_d2h": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_memcpy_d2d": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_fft_pass": (C.c_int, [C.POINTER(hgrid_pass)]),
+    "hgrid_stream_copy": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_pass_blocks": (L, [C.c_int, L, L, L, C.c_int]),
 }
 

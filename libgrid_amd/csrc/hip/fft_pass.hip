@@ -31,6 +31,7 @@
 
 #include "cx.hpp"
 #include "grid_hip.h"
+#include "phase_math.hpp"
 #include "rt.hpp"
 
 namespace lg {
@@ -159,10 +160,15 @@ template <typename C> __device__ __forceinline__ C phase_factor(double cr, doubl
 template <> __device__ __forceinline__ double2 phase_factor<double2>(double cr, double ci, double vr, double vi) {
   const double ar = cr * vr - ci * vi, ai = cr * vi + ci * vr;
   double s, c;
-  sincos(ai, &s, &c);
-  const double m = exp(ar);
+  sincos_f64(ai, &s, &c);
+  const double m = exp_f64(ar);
   return make_double2(m * c, m * s);
 }
+
+// exp(x) for a real exponent: FP64 for double storage, FP32 after an FP64 argument otherwise
+template <typename C> __device__ __forceinline__ double real_decay(double x);
+template <> __device__ __forceinline__ double real_decay<double2>(double x) { return exp_f64(x); }
+template <> __device__ __forceinline__ double real_decay<float2>(double x) { return (double)expf((float)x); }
 
 template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, double ci, double vr, double vi) {
   // FP64 argument, reduced mod 2 pi in FP64, then FP32 sincos / exp
@@ -242,18 +248,22 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = cmul(v[j], s);
     }
-    if (fl & (HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED)) {
+    if (fl & (HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF)) {
       const auto rrho = make_rsrc(a.rho_out, (unsigned long)ntot * 8);
       const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const unsigned e = e0 + (unsigned)j * estep;
         const double d = cnorm2(v[j]);
-        if ((fl & HGRID_PF_PRE_RHO) && active) bstore_f64(d, rrho, e * 8u, 0);
+        const double vx = (a.vext && active) ? bload_f64(rvx, e * 8u, 0) : 0.0;
+        if (active) {
+          if (fl & HGRID_PF_PRE_VEFF) bstore_f64(vx + a.g * d, rrho, e * 8u, 0);
+          else if (fl & HGRID_PF_PRE_RHO) bstore_f64(d, rrho, e * 8u, 0);
+        }
         if (fl & HGRID_PF_PRE_RED) {
           acc[HGRID_RED_NORM] += d;
           acc[HGRID_RED_QUART] += d * d;
-          if (a.vext) acc[HGRID_RED_VEXT] += (active ? bload_f64(rvx, e * 8u, 0) : 0.0) * d;
+          acc[HGRID_RED_VEXT] += vx * d;
         }
       }
     }
@@ -278,19 +288,33 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     const auto rpot = make_rsrc(a.pot, (unsigned long)ntot * CS);
     const auto rvx = make_rsrc(a.vext, (unsigned long)ntot * 8);
     const auto rrho = make_rsrc(a.rho, (unsigned long)ntot * 8);
+    if constexpr (KIND == KIND_POT) {
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const unsigned e = e0 + (unsigned)j * estep;
-      double vr, vi;
-      if constexpr (KIND == KIND_POT) {
-        const C pv = bload<C>(rpot, e * CS, 0);
-        vr = a.cons + (double)pv.x;
-        vi = (double)pv.y;
-      } else {
-        vr = a.cons + bload_f64(rvx, e * 8u, 0) + a.g * bload_f64(rrho, e * 8u, 0);
-        vi = 0.0;
+      for (int j = 0; j < R; ++j) {
+        const C pv = bload<C>(rpot, (e0 + (unsigned)j * estep) * CS, 0);
+        v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, a.cons + (double)pv.x, (double)pv.y));
       }
-      v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, vr, vi));
+    } else {
+      // real potential: V = vext + g rho, or rho alone when it already holds V_eff
+      double vr[R];
+      if (a.vext) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const unsigned e = e0 + (unsigned)j * estep;
+          vr[j] = a.cons + bload_f64(rvx, e * 8u, 0) + a.g * bload_f64(rrho, e * 8u, 0);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) vr[j] = a.cons + bload_f64(rrho, (e0 + (unsigned)j * estep) * 8u, 0);
+      }
+      if (a.c_im == 0.0) {
+        // imaginary time: a real decay factor exp(c_re * V)
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = cscale(v[j], (Rl)((double)mid * real_decay<C>(a.c_re * vr[j])));
+      } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, vr[j], 0.0));
+      }
     }
   }
 

@@ -1,0 +1,90 @@
+// Compact FP64 exp and sin/cos for the phase factors exp(c * V) of the potential step.
+//
+// The general-purpose device math library handles every argument range (Payne-Hanek
+// reduction, special cases) and its inlined code costs more registers than the rest of a
+// fused FFT pass, halving that pass's occupancy.  These versions use textbook methods:
+//   exp:    x = k ln2 + r with a two-part (Cody-Waite) ln2, |r| <= ln2/2, Taylor series to
+//           r^13 (truncation < 1e-17 relative), then 2^k by ldexp.
+//   sincos: x = n pi/2 + r with a three-part pi/2 whose leading parts have 33 significant
+//           bits, so n * part is exact for |n| < 2^20 (|x| < 1.6e6); Taylor series of sin to
+//           r^17 and cos to r^18 on |r| <= pi/4 (truncation < 1e-17); quadrant from n mod 4.
+// Both are within a few ulp of the correctly rounded result over their ranges (checked
+// against glibc on the host by tests/test_phase_math.py).  |x| >= 1.6e6 in sincos loses
+// accuracy gradually; a potential step with |t V / hbar| beyond that is meaningless anyway.
+// Constants were derived to 50+ digits (pi by Machin's formula, ln 2 by atanh series).
+#pragma once
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define LG_HD __host__ __device__ __forceinline__
+#else
+#define LG_HD inline
+#endif
+
+namespace lg {
+
+LG_HD double exp_f64(double x) {
+  if (!(x == x)) return x;
+  if (x > 709.782712893384) return HUGE_VAL;
+  if (x < -745.1332191019412) return 0.0;
+  const double inv_ln2 = 1.4426950408889634;
+  const double ln2_hi = 0.6931471806019545;      // 32 significant bits
+  const double ln2_lo = -4.2009150726810846e-11; // ln 2 - ln2_hi
+  const double k = rint(x * inv_ln2);
+  const double r = (x - k * ln2_hi) - k * ln2_lo;
+  double p = 1.6059043836821613e-10;            // 1/13!
+  p = p * r + 2.08767569878681e-09;             // 1/12!
+  p = p * r + 2.505210838544172e-08;            // 1/11!
+  p = p * r + 2.755731922398589e-07;            // 1/10!
+  p = p * r + 2.7557319223985893e-06;           // 1/9!
+  p = p * r + 2.48015873015873e-05;             // 1/8!
+  p = p * r + 0.0001984126984126984;            // 1/7!
+  p = p * r + 0.001388888888888889;             // 1/6!
+  p = p * r + 0.008333333333333333;             // 1/5!
+  p = p * r + 0.041666666666666664;             // 1/4!
+  p = p * r + 0.16666666666666666;              // 1/3!
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)k);
+}
+
+LG_HD void sincos_f64(double x, double *s, double *c) {
+  if (!(x - x == 0.0)) {  // inf or nan
+    *s = *c = x - x;
+    return;
+  }
+  const double two_over_pi = 0.6366197723675814;
+  const double pio2_hi = 1.5707963267341256;      // 33 significant bits
+  const double pio2_mid = 6.077100506303966e-11;  // 33 significant bits
+  const double pio2_lo = 2.0222662487959506e-21;
+  const double n = rint(x * two_over_pi);
+  const double r = ((x - n * pio2_hi) - n * pio2_mid) - n * pio2_lo;
+  const double r2 = r * r;
+  double ps = 2.8114572543455206e-15;   // 1/17!
+  ps = ps * r2 - 7.647163731819816e-13;  // -1/15!
+  ps = ps * r2 + 1.6059043836821613e-10; // 1/13!
+  ps = ps * r2 - 2.505210838544172e-08;  // -1/11!
+  ps = ps * r2 + 2.7557319223985893e-06; // 1/9!
+  ps = ps * r2 - 0.0001984126984126984;  // -1/7!
+  ps = ps * r2 + 0.008333333333333333;   // 1/5!
+  ps = ps * r2 - 0.16666666666666666;    // -1/3!
+  const double sr = r + r * r2 * ps;
+  double pc = -1.5619206968586225e-16;   // -1/18!
+  pc = pc * r2 + 4.779477332387385e-14;  // 1/16!
+  pc = pc * r2 - 1.1470745597729725e-11; // -1/14!
+  pc = pc * r2 + 2.08767569878681e-09;   // 1/12!
+  pc = pc * r2 - 2.755731922398589e-07;  // -1/10!
+  pc = pc * r2 + 2.48015873015873e-05;   // 1/8!
+  pc = pc * r2 - 0.001388888888888889;   // -1/6!
+  pc = pc * r2 + 0.041666666666666664;   // 1/4!
+  pc = pc * r2 - 0.5;
+  const double cr = 1.0 + r2 * pc;
+  const long long q = (long long)n & 3;
+  if (q == 0) { *s = sr; *c = cr; }
+  else if (q == 1) { *s = cr; *c = -sr; }
+  else if (q == 2) { *s = -sr; *c = -cr; }
+  else { *s = -cr; *c = sr; }
+}
+
+}  // namespace lg

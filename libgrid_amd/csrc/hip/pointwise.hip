@@ -13,6 +13,7 @@
 
 #include "cx.hpp"
 #include "grid_hip.h"
+#include "phase_math.hpp"
 #include "rt.hpp"
 
 namespace lg {
@@ -55,6 +56,10 @@ __global__ void k_runpack(double *d, const double *s, long rows, long nz, long n
     d[i] = k < nz ? s[r * nz + k] : 0.0;
   }
 }
+__global__ void __launch_bounds__(kNT) k_stream_copy(double2 *d, const double2 *s, long n) {
+  LG_GRID_STRIDE(i, n) { d[i] = s[i]; }
+}
+
 __global__ void k_fill_bytes(unsigned char *p, unsigned char v, long n) {
   LG_GRID_STRIDE(i, n) { p[i] = v; }
 }
@@ -85,34 +90,60 @@ template <int NCH> __device__ __forceinline__ void block_sum_store(double (&acc)
   }
 }
 
-// One workgroup: out[c] = sum_b partials[c * nb + b], Neumaier compensation per thread,
-// fixed-order tree across threads.
-__global__ void __launch_bounds__(kNT) k_reduce_partials(double *out, const double *partials, long nb, int nchan) {
-  __shared__ double sh_s[kNT], sh_c[kNT];
-  for (int c = 0; c < nchan; ++c) {
-    double s = 0.0, comp = 0.0;
-    for (long b = threadIdx.x; b < nb; b += kNT) {
-      const double x = partials[(long)c * nb + b];
-      const double t = s + x;
-      if (fabs(s) >= fabs(x)) comp += (s - t) + x;
-      else comp += (x - t) + s;
-      s = t;
+// One workgroup of 1024 threads sums one channel: each thread a fixed strided subset with
+// Neumaier compensation, then a fixed-order tree; bitwise reproducible.
+constexpr int kRT = 1024;
+__device__ double reduce_channel(const double *x, long nb, double *sh_s, double *sh_c) {
+  double s = 0.0, comp = 0.0;
+  for (long b = threadIdx.x; b < nb; b += kRT) {
+    const double v = x[b];
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) comp += (s - t) + v;
+    else comp += (v - t) + s;
+    s = t;
+  }
+  sh_s[threadIdx.x] = s;
+  sh_c[threadIdx.x] = comp;
+  __syncthreads();
+  for (int w = kRT / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double a = sh_s[threadIdx.x], v = sh_s[threadIdx.x + w];
+      const double t = a + v;
+      const double e = fabs(a) >= fabs(v) ? (a - t) + v : (v - t) + a;
+      sh_s[threadIdx.x] = t;
+      sh_c[threadIdx.x] += sh_c[threadIdx.x + w] + e;
     }
-    sh_s[threadIdx.x] = s;
-    sh_c[threadIdx.x] = comp;
     __syncthreads();
-    for (int w = kNT / 2; w >= 1; w >>= 1) {
-      if ((int)threadIdx.x < w) {
-        const double a = sh_s[threadIdx.x], x = sh_s[threadIdx.x + w];
-        const double t = a + x;
-        const double e = fabs(a) >= fabs(x) ? (a - t) + x : (x - t) + a;
-        sh_s[threadIdx.x] = t;
-        sh_c[threadIdx.x] += sh_c[threadIdx.x + w] + e;
+  }
+  const double r = sh_s[0] + sh_c[0];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup per reduced channel (blockIdx.x enumerates the channels of all jobs in
+// order).  The workgroup that owns the channel holding the norm sum also writes the scale.
+__global__ void __launch_bounds__(kRT) k_step_finalize(const hgrid_finalize f) {
+  __shared__ double sh_s[kRT], sh_c[kRT];
+  int b = blockIdx.x;
+  for (int j = 0; j < f.njobs; ++j) {
+    const hgrid_reduce_job jb = f.job[j];
+    if (b < jb.nchan) {
+      const double r = reduce_channel(jb.partials + (long)(jb.chan_first + b) * jb.nblocks, jb.nblocks, sh_s, sh_c);
+      if (threadIdx.x == 0) {
+        jb.out[b] = r;
+        if (f.scale && f.target > 0.0 && jb.out + b == f.norm_sum) {
+          f.scale[0] = sqrt(f.target / (r * f.dv));
+          f.scale[1] = 0.0;
+        }
       }
-      __syncthreads();
+      return;
     }
-    if (threadIdx.x == 0) out[c] = sh_s[0] + sh_c[0];
-    __syncthreads();
+    b -= jb.nchan;
+  }
+  // the extra workgroup (no channel left): reset a consumed pending scale
+  if (threadIdx.x == 0 && f.scale && !(f.target > 0.0)) {
+    f.scale[0] = 1.0;
+    f.scale[1] = 0.0;
   }
 }
 
@@ -213,8 +244,8 @@ template <typename C> __device__ __forceinline__ C pfac(double cr, double ci, do
 template <> __device__ __forceinline__ double2 pfac<double2>(double cr, double ci, double vr, double vi) {
   const double ar = cr * vr - ci * vi, ai = cr * vi + ci * vr;
   double s, c;
-  sincos(ai, &s, &c);
-  const double m = exp(ar);
+  sincos_f64(ai, &s, &c);
+  const double m = exp_f64(ar);
   return make_double2(m * c, m * s);
 }
 template <> __device__ __forceinline__ float2 pfac<float2>(double cr, double ci, double vr, double vi) {
@@ -317,13 +348,42 @@ extern "C" int hgrid_rpack(double *d, const double *s, long rows, long nz, long 
 extern "C" int hgrid_runpack(double *d, const double *s, long rows, long nz, long nz2) {
   LG_LAUNCH(k_runpack, rows * nz2, d, s, rows, nz, nz2);
 }
+extern "C" int hgrid_stream_copy(void *d, const void *s, size_t bytes) {
+  const long n = (long)(bytes / 16);
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid_for(n, 16)), dim3(kNT), 0, stream(), (double2 *)d, (const double2 *)s, n);
+  return check_launch("stream_copy");
+}
+
 extern "C" int hgrid_memset(void *p, int v, size_t bytes) {
   LG_LAUNCH(k_fill_bytes, (long)bytes, (unsigned char *)p, (unsigned char)v, (long)bytes);
 }
 
+extern "C" int hgrid_step_finalize(const hgrid_finalize *f) {
+  if (f->njobs < 0 || f->njobs > HGRID_MAX_JOBS) return -1;
+  int nb = 0;
+  for (int j = 0; j < f->njobs; ++j) nb += f->job[j].nchan;
+  if (f->scale && f->target > 0.0) {
+    /* the norm sum must be one of the reduced outputs */
+    int found = 0;
+    for (int j = 0; j < f->njobs; ++j)
+      for (int c = 0; c < f->job[j].nchan; ++c) found |= (f->job[j].out + c == f->norm_sum);
+    if (!found) return -1;
+  }
+  if (f->scale && !(f->target > 0.0)) nb += 1;
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL(k_step_finalize, dim3(nb), dim3(kRT), 0, stream(), *f);
+  return check_launch("step_finalize");
+}
+
 extern "C" int hgrid_reduce_partials(double *out, const double *partials, long nb, int nchan) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kNT), 0, stream(), out, partials, nb, nchan);
-  return check_launch("reduce_partials");
+  hgrid_finalize f = {};
+  f.njobs = 1;
+  f.job[0].partials = partials;
+  f.job[0].nblocks = nb;
+  f.job[0].chan_first = 0;
+  f.job[0].nchan = nchan;
+  f.job[0].out = out;
+  return hgrid_step_finalize(&f);
 }
 
 extern "C" long hgrid_reduce_scratch(long n) { return 2L * (long)grid_for(n) + 2; }

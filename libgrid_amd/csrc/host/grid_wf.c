@@ -209,6 +209,7 @@ typedef struct {
   double tm_re, tm_im;   /* time_mass of the half step */
   int want_energy;       /* reductions for E of the input state */
   int want_post_norm;    /* sum |psi|^2 of the output */
+  double renorm_target;  /* > 0: make the output's pending scale renormalise it to this norm */
 } step_args;
 
 static void fused_step(wf *gwf, const step_args *sa) {
@@ -232,10 +233,22 @@ static void fused_step(wf *gwf, const step_args *sa) {
   /* P1: forward along the contiguous axis; pending scale, density and input reductions */
   lg_pass_init(&p, &s, prec, psi, psi, 2);
   if (scale) { p.flags |= HGRID_PF_PRESCALE; p.scale = scale; }
-  if (sa->gpe) { p.flags |= HGRID_PF_PRE_RHO; p.rho_out = d->rho; }
+  if (sa->gpe) {
+    /* V_eff = vext + g |psi|^2 of the start-of-step state, read back by the potential pass */
+    p.flags |= HGRID_PF_PRE_VEFF;
+    p.rho_out = d->rho;
+    p.vext = sa->vext;
+    p.g = sa->g;
+  }
   if (sa->want_energy) { p.flags |= HGRID_PF_PRE_RED; p.vext = sa->vext; p.partials = part1; }
   lg_run_pass(&p);
-  if (sa->want_energy) lg_hip(hgrid_reduce_partials(d->dscal + GS_RED, part1, nb2, 3), "reduce");
+  /* all reductions of the step and the pending-scale update go into one finalize launch */
+  hgrid_finalize fin;
+  memset(&fin, 0, sizeof(fin));
+  if (sa->want_energy) {
+    hgrid_reduce_job j = {part1, nb2, HGRID_RED_NORM, 3, d->dscal + GS_RED};
+    fin.job[fin.njobs++] = j;
+  }
 
   for (int half = 0; half < 2; ++half) {
     for (int axis = 1; axis > aK; --axis) {
@@ -255,8 +268,10 @@ static void fused_step(wf *gwf, const step_args *sa) {
       p.partials = partK;
     }
     lg_run_pass(&p);
-    if (half == 0 && sa->want_energy)
-      lg_hip(hgrid_reduce_partials(d->dscal + GS_RED + HGRID_RED_KIN, partK + HGRID_RED_KIN * nbK, nbK, 1), "reduce");
+    if (half == 0 && sa->want_energy) {
+      hgrid_reduce_job j = {partK, nbK, HGRID_RED_KIN, 1, d->dscal + GS_RED + HGRID_RED_KIN};
+      fin.job[fin.njobs++] = j;
+    }
     for (int axis = aK + 1; axis <= 1; ++axis) {
       lg_pass_init(&p, &s, prec, psi, psi, axis);
       p.dir1 = +1;
@@ -275,9 +290,8 @@ static void fused_step(wf *gwf, const step_args *sa) {
         p.mid = 1.0;
         if (sa->gpe) {
           p.flags |= HGRID_PF_MID_GPE;
-          p.vext = sa->vext;
+          p.vext = NULL; /* d->rho holds V_eff */
           p.rho = d->rho;
-          p.g = sa->g;
         } else {
           p.flags |= HGRID_PF_MID_POT;
           p.pot = sa->pot;
@@ -298,9 +312,21 @@ static void fused_step(wf *gwf, const step_args *sa) {
   p.dir1 = +1;
   if (sa->want_post_norm) { p.flags |= HGRID_PF_POST_RED; p.partials = part9; }
   lg_run_pass(&p);
-  if (sa->want_post_norm)
-    lg_hip(hgrid_reduce_partials(d->dscal + GS_RED + HGRID_RED_POST, part9 + HGRID_RED_POST * nb2, nb2, 1), "reduce");
-  lg_c_written(g, 1);
+  if (sa->want_post_norm) {
+    hgrid_reduce_job j = {part9, nb2, HGRID_RED_POST, 1, d->dscal + GS_RED + HGRID_RED_POST};
+    fin.job[fin.njobs++] = j;
+  }
+  if (sa->renorm_target > 0.0) {
+    fin.scale = d->dscal + GS_SCALE;
+    fin.norm_sum = d->dscal + GS_RED + HGRID_RED_POST;
+    fin.target = sa->renorm_target;
+    fin.dv = lg_step_factor(g->nx, g->ny, g->nz, g->step);
+  } else if (scale) {
+    fin.scale = d->dscal + GS_SCALE; /* the pending scale was applied by the first pass */
+  }
+  if (fin.njobs || fin.scale) lg_hip(hgrid_step_finalize(&fin), "step_finalize");
+  d->valid = GV_DEV;
+  d->pending = sa->renorm_target > 0.0;
 }
 
 /* unfused sequence for grids with fewer than two non-trivial axes (reference order) */
@@ -474,14 +500,9 @@ void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, c
   sa.tm_im = cimag(tm);
   sa.want_energy = 1;
   sa.want_post_norm = normalize ? 1 : 0;
+  sa.renorm_target = normalize ? gwf->norm : 0.0;
   fused_step(gwf, &sa);
   grid_dev *d = g->dev;
-  if (normalize) {
-    lg_hip(hgrid_scalar_renorm(d->dscal + GS_SCALE, d->dscal + GS_RED + HGRID_RED_POST, gwf->norm,
-                               lg_step_factor(g->nx, g->ny, g->nz, g->step)),
-           "scalar_renorm");
-    d->pending = 1;
-  }
   d->have_gpe_energy = 1;
   g_last_g = g_nl;
 }
