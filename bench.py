@@ -137,6 +137,43 @@ def profiled_traffic(pass_name, n):
     return None
 
 
+def ground_state(lib, LG, psi, vext, prm, n, max_steps):
+    """Wall time to |E_n - E_{n-1}| < 1e-10 Eh from the workload's initial state:
+    (a) FP64 throughout; (b) FP32 storage until the energy change falls below 1e-7 Eh (the
+    FP32 noise floor is ~1e-8), then FP64 to 1e-10.  Energies of both runs are reported."""
+    step, g, dt, tol = prm["step"], prm["g"], prm["dt"], prm["tol"]
+    out = {}
+    for mode in ("fp64", "fp32_then_fp64"):
+        w = lib.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"gs")
+        vr = lib.rgrid_alloc(n, n, n, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vext")
+        LG.set_cgrid(w.contents.grid, psi)
+        LG.set_rgrid(vr, vext)
+        e = C.c_double()
+        # upload and table set-up outside the timed region
+        lib.grid_wf_gpe_energy(w, vr, g)
+        if mode == "fp32_then_fp64":
+            lib.cgrid_set_precision(w.contents.grid, b"\x01")
+        lib.grid_hip_sync()
+        t0 = time.perf_counter()
+        if mode == "fp64":
+            steps = lib.grid_wf_gpe_relax(w, vr, g, dt, tol, max_steps, C.byref(e))
+            s32 = 0
+        else:
+            s32 = lib.grid_wf_gpe_relax(w, vr, g, dt, 1e-7, max_steps, C.byref(e))
+            lib.cgrid_set_precision(w.contents.grid, b"\x00")
+            steps = lib.grid_wf_gpe_relax(w, vr, g, dt, tol, max_steps, C.byref(e))
+        lib.grid_hip_sync()
+        el = time.perf_counter() - t0
+        efinal = lib.grid_wf_gpe_energy(w, vr, g)
+        out[mode] = dict(seconds=el, fp64_steps=steps, fp32_steps=s32, energy=efinal, converged=steps < max_steps)
+        log(f"ground state [{mode}]: {el:.3f} s, fp32 steps {s32}, fp64 steps {steps}, E = {efinal:.12f}")
+        lib.grid_wf_free(w)
+        lib.rgrid_free(vr)
+    out["speedup_fp32_then_fp64"] = out["fp64"]["seconds"] / out["fp32_then_fp64"]["seconds"]
+    out["tol_Eh"] = tol
+    return out
+
+
 def cpu_baseline(psi, vext, prm, steps):
     from oracle import oracle as orc
 
@@ -218,6 +255,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gs", action="store_true", help="skip the time-to-ground-state runs")
+    ap.add_argument("--gs-max-steps", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -271,6 +310,10 @@ def main():
 
     traffic = profiled_traffic(dname, n)
 
+    gs = None
+    if not args.no_gs:
+        gs = ground_state(lib, LG, psi, vext, prm, n, args.gs_max_steps)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -304,6 +347,7 @@ def main():
         "pass_ms": {k: v["ms"] for k, v in kinds.items()},
         "copy_ceiling_GBps": copy["gbps"],
         "energy": energy,
+        "time_to_ground_state": gs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

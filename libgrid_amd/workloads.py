@@ -10,6 +10,11 @@ Workloads:
   gpe_barrier          256^3 complex128, step 0.25: V = 0.5*0.01*r^2 + 0.05*exp(-r^2/2^2), g = 0.1,
                        Gaussian (sigma = 1/sqrt(m*omega)) times exp(i theta), theta ~ U[0, 2pi) per
                        point then a 3x3x3 periodic box average; imaginary-time relaxation.
+  vortex_ring          512^3, step 0.15: condensate envelope exp(-r^2 / (2 sigma^2)), sigma = 10, in
+                       V = 0.5*0.01*r^2 with g = 0.1, times a vortex ring of radius R ~ U[4, 8] bohr
+                       about an axis drawn uniformly on the sphere (core profile tanh(d / xi),
+                       xi = 1, phase winding atan2(axial offset, radial offset - R)); real time
+                       dt = 0.02, 500 steps.
 """
 from __future__ import annotations
 
@@ -80,4 +85,24 @@ def gpe_barrier(seed: int = 0, n: int = 256, step: float = 0.25, omega: float = 
     psi = normalize(psi.astype(np.complex128), step)
     params = dict(shape=shape, step=step, mass=1.0, g=g, omega=omega, barrier=barrier, barrier_width=barrier_width,
                   dt=0.05, tol=1e-10)
+    return psi, vext, params
+
+
+def vortex_ring(seed: int = 0, n: int = 512, step: float = 0.15, sigma: float = 10.0, xi: float = 1.0,
+                omega: float = 0.1, g: float = 0.1):
+    rng = np.random.default_rng(seed)
+    radius = float(rng.uniform(4.0, 8.0))
+    v = rng.standard_normal(3)
+    axis = v / np.linalg.norm(v)
+    shape = (n, n, n)
+    x, y, z = coords(shape, step)
+    r2 = x * x + y * y + z * z
+    zp = x * axis[0] + y * axis[1] + z * axis[2]               # axial coordinate
+    rho = np.sqrt(np.maximum(r2 - zp * zp, 0.0))               # distance from the axis
+    d = np.sqrt((rho - radius) ** 2 + zp * zp)                 # distance from the ring core
+    psi = np.exp(-r2 / (2 * sigma * sigma)) * np.tanh(d / xi) * np.exp(1j * np.arctan2(zp, rho - radius))
+    psi = normalize(psi.astype(np.complex128), step)
+    vext = np.broadcast_to((0.5 * omega**2 * r2).astype(np.float64), shape).copy()
+    params = dict(shape=shape, step=step, mass=1.0, g=g, omega=omega, radius=radius, axis=axis.tolist(), sigma=sigma,
+                  xi=xi, dt=0.02, steps=500)
     return psi, vext, params

@@ -1,0 +1,112 @@
+"""The workloads of the benchmark suite as GPU parity cases (full sizes).
+
+  wavepacket 128^3 FP64: 100 real-time steps vs the oracle, norm every step, final FFT energy;
+  gpe barrier 256^3 FP64: the first imaginary-time GPE steps vs the oracle's composition;
+  vortex ring 512^3 FP32 (FP64 twiddles / arguments / reductions): 500 real-time GPE steps vs
+      the FP64 GPU path on the same inputs (the FP64 path is pinned to the oracle above and in
+      test_gpu_parity.py); reports relative L2, norm drift and energy drift.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import libgrid_amd as LG
+from conftest import rel_l2
+from libgrid_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_wavepacket_128_full(lg, orc):
+    psi0, v, prm = W.wavepacket_harmonic(seed=0)
+    case(f"wavepacket 128^3 100 steps {prm}")
+    step, mass, dt = prm["step"], prm["mass"], prm["dt"]
+    pot = v.astype(np.complex128)
+    w = lg.grid_wf_alloc(128, 128, 128, step, mass, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"wp")
+    LG.set_cgrid(w.contents.grid, psi0)
+    vp = lg.cgrid_alloc(128, 128, 128, step, LG.boundary("cgrid_value_outside_periodic"), None, b"v")
+    LG.set_cgrid(vp, pot)
+    vr = lg.rgrid_alloc(128, 128, 128, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vr")
+    LG.set_rgrid(vr, v)
+    ref = psi0.copy()
+    n_prev = lg.grid_wf_norm(w)
+    worst = 0.0
+    for s in range(prm["steps"]):
+        lg.grid_wf_propagate(w, vp, LG.c_complex.of(dt))
+        orc.wf_propagate_inplace(ref, pot, step, mass, (0, 0, 0), dt)
+        n = lg.grid_wf_norm(w)
+        worst = max(worst, abs(n - n_prev))
+        n_prev = n
+    err = rel_l2(LG.get_cgrid(w.contents.grid), ref)
+    lg.grid_wf_analyze_method(b"\x01")
+    e = lg.grid_wf_energy(w, vr)
+    e_ref = orc.energy_fft(ref, v, step, mass)
+    print(f"wavepacket: rel L2 {err:.2e}, worst per-step norm change {worst:.2e}, E {e:.15f} vs {e_ref:.15f}")
+    assert err < 1e-12
+    assert worst <= 1e-13
+    assert abs(e - e_ref) < 1e-12 * abs(e_ref)
+    lg.grid_wf_free(w)
+    lg.cgrid_free(vp)
+    lg.rgrid_free(vr)
+
+
+def test_gpe_barrier_256_first_steps(lg, orc):
+    psi0, vext, prm = W.gpe_barrier(seed=0)
+    case(f"gpe barrier 256^3, 5 steps {prm}")
+    step, g, dt = prm["step"], prm["g"], prm["dt"]
+    w = lg.grid_wf_alloc(256, 256, 256, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"gpe")
+    LG.set_cgrid(w.contents.grid, psi0)
+    vr = lg.rgrid_alloc(256, 256, 256, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vx")
+    LG.set_rgrid(vr, vext)
+    ref = psi0.copy()
+    for s in range(5):
+        e_ref = orc.gpe_energy(ref, vext, g, step, 1.0)
+        lg.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(-1j * dt), b"\x01")
+        e = lg.grid_wf_gpe_last_energy(w)
+        orc.gpe_step_inplace(ref, vext, g, step, 1.0, (0, 0, 0), -1j * dt, normalize=True)
+        assert abs(e - e_ref) < 1e-12 * abs(e_ref), s
+    err = rel_l2(LG.get_cgrid(w.contents.grid), ref)
+    print(f"gpe 256^3: rel L2 after 5 steps {err:.2e}")
+    assert err < 1e-12
+    lg.grid_wf_free(w)
+    lg.rgrid_free(vr)
+
+
+@pytest.mark.timeout(400)
+def test_vortex_ring_512_fp32_vs_fp64(lg):
+    psi0, vext, prm = W.vortex_ring(seed=0)
+    n = prm["shape"][0]
+    case(f"vortex ring {n}^3 fp32 vs fp64, {prm['steps']} steps, params {prm}")
+    step, g, dt = prm["step"], prm["g"], prm["dt"]
+    vr = lg.rgrid_alloc(n, n, n, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vx")
+    LG.set_rgrid(vr, vext)
+    runs = {}
+    for prec in (b"\x00", b"\x01"):
+        w = lg.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"ring")
+        LG.set_cgrid(w.contents.grid, psi0)
+        lg.cgrid_set_precision(w.contents.grid, prec)
+        n0 = lg.grid_wf_norm(w)
+        energies = []
+        for s in range(prm["steps"]):
+            lg.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(dt), b"\x00")
+            if s == 0 or s == prm["steps"] - 1:
+                energies.append(lg.grid_wf_gpe_last_energy(w))
+        e_end = lg.grid_wf_gpe_energy(w, vr, g)
+        runs[prec] = dict(psi=LG.get_cgrid(w.contents.grid), norm_drift=abs(lg.grid_wf_norm(w) - n0),
+                          e0=energies[0], e_end=e_end)
+        lg.grid_wf_free(w)
+    lg.rgrid_free(vr)
+    d64, d32 = runs[b"\x00"], runs[b"\x01"]
+    err = rel_l2(d32["psi"], d64["psi"])
+    print(f"vortex ring 512^3 after {prm['steps']} steps: fp32 vs fp64 rel L2 {err:.3e}; norm drift fp64 "
+          f"{d64['norm_drift']:.2e} fp32 {d32['norm_drift']:.2e}; energy drift fp64 {d64['e_end'] - d64['e0']:.3e} "
+          f"fp32 {d32['e_end'] - d32['e0']:.3e}; E0 {d64['e0']:.12f}")
+    assert d64["norm_drift"] < 1e-11
+    assert err < 1e-3
+    assert d32["norm_drift"] < 1e-3
