@@ -247,8 +247,38 @@ void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g, REAL complex time, char
 REAL grid_wf_gpe_last_energy(wf *gwf);
 /* E_GP of the current state (FFT kinetic energy, no k0 shift, as grid_wf_kinetic_energy_fft). */
 REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g);
-/* Imaginary-time relaxation with renormalisation every step until |E_n - E_{n-1}| < tol or
- * max_steps; returns the number of steps taken and stores the last energy in *energy. */
+/* Imaginary-time relaxation with renormalisation every step until the energy is converged to
+ * tol, or max_steps.  Converged: with energies sampled every m steps (m doubling from 4 while
+ * the decay is slow) the last decrease D2 is below tol and so is the geometric estimate
+ * D2 q / (1 - q), q = D2 / D1, of the energy still to come; a rise smaller than tol (rounding)
+ * also ends it.  Returns the number of steps taken and stores the last energy in *energy. */
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_steps, REAL *energy);
+
+/* ---- batched 2-D ground states (this build's addition) ----------------------------------
+ * Relax nsolves independent 1 x n x n wavefunctions (n in {64, 96, 128, 192, 256}) in
+ * imaginary time with their potentials (rgrid, same shape) in one persistent GPU launch:
+ * complex64 storage, FP64 twiddles / tables / reductions.  Each solve runs 2nd-order FFT
+ * steps of length dt, renormalised every step, until its FFT energy is converged to tol by the
+ * rule of grid_wf_gpe_relax (or max_steps).  The normalised states are written back to gwf[i]->grid; energy[i]
+ * and steps[i] (may be NULL) receive the final energy and the step at which tol was met
+ * (max_steps if never).  Returns the GPU time of the launch in seconds.  The first call on a
+ * device runs grid_batch2d_autotune unless a tuned tile is cached. */
+REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, REAL tol, INT max_steps, REAL *energy,
+                           INT *steps);
+/* Time every tile configuration of the batched kernel on these problems for probe_steps steps,
+ * keep the fastest (also cached in $XDG_CACHE_HOME or ~/.cache/libgrid_amd/batch2d_tile.txt)
+ * and return a one-line report. */
+const char *grid_batch2d_autotune(INT nsolves, wf **gwf, rgrid **potential, REAL dt, INT probe_steps);
+/* Start and end (seconds from the earliest start, device clock) of every solve of the last
+ * grid_wf_batch2d_relax call, in its input order; returns the count.  Arrays may be NULL. */
+INT grid_batch2d_solve_times(REAL *start, REAL *end);
+/* Record the energy entering each of the first len steps of every solve in later
+ * grid_wf_batch2d_relax calls (0 turns it off); grid_batch2d_history copies solve's record of
+ * the last call (at most len values, entries past its last step undefined) and returns the
+ * number copied. */
+void grid_batch2d_record_history(INT len);
+INT grid_batch2d_history(INT solve, REAL *energy, INT len);
+void grid_batch2d_set_tile(int variant, int nthreads, int per_cu);
+void grid_batch2d_get_tile(int *variant, int *nthreads, int *per_cu);
 
 #endif

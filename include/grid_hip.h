@@ -185,6 +185,36 @@ int hgrid_scalar_renorm(double *s, const double *sum, double target, double dv);
 int hgrid_fd_laplace_expect(double *out, const void *psi, int prec, long nx, long ny, long nz, int bc,
                             const double *scale, double *scratch);
 
+/* ---- batched 2-D imaginary-time solves ---------------------------------------------------- */
+
+/* One n x n solve of the persistent batched kernel (complex64 storage, FP64 reductions). */
+typedef struct hgrid_b2d_job {
+  void *psi;           /* n*n float2, row-major [y][x]; in: start state, out: normalised ground state */
+  const float *pv;     /* n*n exp(-dt V) */
+  const double *v;     /* n*n V (potential energy) */
+  const float *kt;     /* n: exp(-(dt/2) hbar k_m^2 / 2m) / n, the same table for both axes */
+  const double *k2;    /* n: k_m^2 */
+  const void *tw;      /* n: double2 exp(-2 pi i m / n) */
+  double dA;           /* step^2 */
+  double tfac;         /* hbar^2/(2m) * step^2 / n^2 */
+  double tol;          /* energy tolerance of the stopping rule (see grid_wf_gpe_relax) */
+  int n, max_steps;
+  int hist_len;        /* energies of the first hist_len steps go to hist (0: none) */
+  double *hist;
+  double *out;         /* device, HGRID_B2D_NOUT doubles: energy, steps taken, step at which tol
+                          was met (max_steps if never), start and end of the solve on the
+                          device's constant 100 MHz clock */
+} hgrid_b2d_job;
+#define HGRID_B2D_NOUT 5
+
+int hgrid_b2d_supported(long n);   /* n in {64, 96, 128, 192, 256} */
+int hgrid_b2d_cus(void);            /* compute units of the device */
+/* jobs: device array; queue: device int (reset by the call).  Tile choices: variant 0/1
+ * (threads per pencil for n = 64/128/256), nthreads 256/512 per workgroup (pencils per
+ * sweep chunk = nthreads / threads per pencil), nblocks workgroups (persistent: each takes
+ * solves from the queue until it is empty). */
+int hgrid_b2d_launch(const hgrid_b2d_job *jobs, int njobs, int *queue, int variant, int nthreads, int nblocks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -263,6 +263,13 @@ PROTOTYPES = {
     "grid_wf_gpe_last_energy": (D, [WF]),
     "grid_wf_gpe_energy": (D, [WF, RG, D]),
     "grid_wf_gpe_relax": (L, [WF, RG, D, D, D, L, C.POINTER(D)]),
+    "grid_wf_batch2d_relax": (D, [L, C.POINTER(WF), C.POINTER(RG), D, D, L, C.POINTER(D), C.POINTER(L)]),
+    "grid_batch2d_autotune": (C.c_char_p, [L, C.POINTER(WF), C.POINTER(RG), D, L]),
+    "grid_batch2d_solve_times": (L, [C.POINTER(D), C.POINTER(D)]),
+    "grid_batch2d_record_history": (V, [L]),
+    "grid_batch2d_history": (L, [L, C.POINTER(D), L]),
+    "grid_batch2d_set_tile": (V, [C.c_int, C.c_int, C.c_int]),
+    "grid_batch2d_get_tile": (V, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     # thin C-ABI (grid_hip.h) used for timing
     "hgrid_init": (C.c_int, [C.c_int]),
     "hgrid_last_error": (C.c_char_p, []),

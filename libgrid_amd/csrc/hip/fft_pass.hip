@@ -1,421 +1,13 @@
-// Pencil FFT passes for gfx950: one pass transforms every pencil of one axis of a 3-D
-// complex array and may fuse pointwise work in front of, between and behind its transforms.
-//
-// Decomposition of one pencil of length L = R * TPP (TPP threads per pencil, R values per
-// thread).  Thread t holds x[t + TPP*j], j < R.  With n = t + TPP*j and k = k1 + R*k2:
-//     X[k1 + R*k2] = sum_t W_TPP^{t*k2} * ( W_L^{t*k1} * sum_j x[t + TPP*j] W_R^{j*k1} )
-// stage 1 is an R-point DFT in registers, then a twiddle W_L^{t*k1} from an FP64-accurate
-// table, an LDS transpose, and stage 2: R/TPP TPP-point DFTs per thread.  The result lands
-// in the same distribution as the input (thread t holds X[t + TPP*m]), so a second
-// transform (inverse after forward, or forward after inverse) can follow in registers.
-//
-// Memory layout (squeezed shape n0 x n1 x n2, n2 contiguous):
-//   axis 2 ("contiguous" pass): pencil r = i0*n1 + i1 starts at r*L, stride 1.  Lanes of one
-//            pencil are adjacent, so one wave reads TPP*sizeof(C) runs of consecutive bytes.
-//   axis 1/0 ("strided" pass): pencils are flattened over (outer, i2); a workgroup takes P
-//            consecutive i2 values so a wave load covers P*sizeof(C) contiguous bytes per row.
-// Global traffic uses raw buffer instructions: one 32-bit VGPR offset per thread plus a
-// uniform SGPR offset per element, instead of a 64-bit address per element.
-//
-// Pass kinds (compile time, so each kernel carries only the code it runs):
-//   KIND_FFT : one transform; optional pending scale on load, |psi|^2 / reductions on the
-//              input, real post factor and |psi|^2 reduction on the output
-//   KIND_KIN : forward, kinetic factor (optionally sum k^2|psi_k|^2 first), inverse
-//   KIND_POT : inverse, mid * exp(c (cons + V)) with complex V, forward
-//   KIND_GPE : inverse, mid * exp(c (cons + vext + g rho)), forward
-// Inverse transforms use conj(DFT(conj(x))), so one set of butterflies serves both signs.
-// Reference operators this replaces: cgrid_fft / cgrid_inverse_fft (src/cgrid.c:1822-1846,
-// src/cgrid_fft.c:101-146), the kinetic phase (src/grid_wf_fft.c:141-193), the potential
-// phase (src/grid_wf.c:569-600), the 1/N scaling (src/cgrid.c:1857-1876) and the reductions
-// (src/cgrid.c:1610-1629, src/cgrid_diff.c:817-879, src/ngrid.c:355-376).
+// C-ABI of the pencil FFT passes (see fft_pass.hpp for the kernels).
+#include "fft_pass.hpp"
 
-#include "cx.hpp"
-#include "grid_hip.h"
-#include "phase_math.hpp"
-#include "rt.hpp"
-
-namespace lg {
-
-enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3 };
-
-
-template <int L> struct Split {
-  static constexpr int TPP = L <= 16 ? 1 : L == 32 ? 4 : L == 64 ? 8 : L == 128 ? 8 : L == 256 ? 16 : L == 512 ? 16 : 32;
-  static constexpr int R = L / TPP;
-  static_assert(R % TPP == 0, "stage 2 needs R to be a multiple of TPP");
-};
-
-template <typename C, int L> struct Geom {
-  static constexpr int TPP = Split<L>::TPP;
-  static constexpr int R = Split<L>::R;
-  // workgroup size: 256 threads unless one real half of the LDS exchange exceeds 32 KiB
-  static constexpr int NT = (TPP > 1 && (long)R * (long)sizeof(C) * 128 > 32768) ? 128 : 256;
-  static constexpr int P = NT / TPP;  // pencils per workgroup
-  // the exchange moves real parts, then imaginary parts, through one real-valued buffer
-  static constexpr int LDS_CONTIG = TPP > 1 ? P * R * (TPP + 1) : 1;
-  static constexpr int LDS_STRIDED = TPP > 1 ? P * R * TPP : 1;
-};
-
-template <bool CONTIG, int R, int TPP, int P>
-__device__ __forceinline__ int lds_index(int q, int k1, int t) {
-  if constexpr (CONTIG) return q * (R * (TPP + 1)) + k1 * (TPP + 1) + t;
-  else return (k1 * TPP + t) * P + q;
-}
-
-// ---- raw buffer access ---------------------------------------------------------------------
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned long bytes) {
-  const int num = bytes >= 0xffffffffUL ? (int)0xffffffff : (int)bytes;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, num, 0x00020000);
-}
-template <typename C> __device__ __forceinline__ C bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
-template <> __device__ __forceinline__ double2 bload<double2>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-  double2 d;
-  __builtin_memcpy(&d, &x, 16);
-  return d;
-}
-template <> __device__ __forceinline__ float2 bload<float2>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  float2 d;
-  __builtin_memcpy(&d, &x, 8);
-  return d;
-}
-__device__ __forceinline__ double bload_f64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  double d;
-  __builtin_memcpy(&d, &x, 8);
-  return d;
-}
-template <typename C> __device__ __forceinline__ void bstore(C v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
-template <> __device__ __forceinline__ void bstore<double2>(double2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  __attribute__((ext_vector_type(4))) unsigned x;
-  __builtin_memcpy(&x, &v, 16);
-  __builtin_amdgcn_raw_buffer_store_b128(x, r, voff, soff, 0);
-}
-template <> __device__ __forceinline__ void bstore<float2>(float2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  __attribute__((ext_vector_type(2))) unsigned x;
-  __builtin_memcpy(&x, &v, 8);
-  __builtin_amdgcn_raw_buffer_store_b64(x, r, voff, soff, 0);
-}
-__device__ __forceinline__ void bstore_f64(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  __attribute__((ext_vector_type(2))) unsigned x;
-  __builtin_memcpy(&x, &v, 8);
-  __builtin_amdgcn_raw_buffer_store_b64(x, r, voff, soff, 0);
-}
-
-// ---- one L-point transform of the pencil owned by (q, t) ----------------------------------
-
-template <typename C, int L, bool CONTIG>
-__device__ __forceinline__ void fft_core(C (&v)[Split<L>::R], int t, int q, bool inverse, const C *__restrict__ tw,
-                                         typename real_of<C>::type *lds) {
-  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, P = Geom<C, L>::P;
-  if (inverse) {
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cconj(v[j]);
-  }
-  dft<R>(v);
-  if constexpr (TPP > 1) {
-    // Hide the table address from the optimiser so the twiddles of a second transform in the
-    // same pass are reloaded (L1 hits) instead of being kept live across the mid-pass work.
-    const C *twl = tw;
-    __asm__ volatile("" : "+s"(twl));
-#pragma unroll
-    for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], twl[t * k1]);
-    constexpr int A = R / TPP;
-    // real parts
-    __syncthreads();  // the previous exchange may still be reading
-#pragma unroll
-    for (int k1 = 0; k1 < R; ++k1) lds[lds_index<CONTIG, R, TPP, P>(q, k1, t)] = v[k1].x;
-    __syncthreads();
-    typename real_of<C>::type ur[A][TPP];
-#pragma unroll
-    for (int a = 0; a < A; ++a)
-#pragma unroll
-      for (int s = 0; s < TPP; ++s) ur[a][s] = lds[lds_index<CONTIG, R, TPP, P>(q, t + TPP * a, s)];
-    __syncthreads();
-    // imaginary parts
-#pragma unroll
-    for (int k1 = 0; k1 < R; ++k1) lds[lds_index<CONTIG, R, TPP, P>(q, k1, t)] = v[k1].y;
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < A; ++a) {
-      C u[TPP];
-#pragma unroll
-      for (int s = 0; s < TPP; ++s) u[s] = mk<C>(ur[a][s], lds[lds_index<CONTIG, R, TPP, P>(q, t + TPP * a, s)]);
-      dft<TPP>(u);
-#pragma unroll
-      for (int k2 = 0; k2 < TPP; ++k2) v[a + A * k2] = u[k2];
-    }
-  }
-  if (inverse) {
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cconj(v[j]);
-  }
-}
-
-// exp(c * V) with complex c and complex V (FP64 argument).
-template <typename C> __device__ __forceinline__ C phase_factor(double cr, double ci, double vr, double vi);
-
-template <> __device__ __forceinline__ double2 phase_factor<double2>(double cr, double ci, double vr, double vi) {
-  const double ar = cr * vr - ci * vi, ai = cr * vi + ci * vr;
-  double s, c;
-  sincos_f64(ai, &s, &c);
-  const double m = exp_f64(ar);
-  return make_double2(m * c, m * s);
-}
-
-// exp(x) for a real exponent: FP64 for double storage, FP32 after an FP64 argument otherwise
-template <typename C> __device__ __forceinline__ double real_decay(double x);
-template <> __device__ __forceinline__ double real_decay<double2>(double x) { return exp_f64(x); }
-template <> __device__ __forceinline__ double real_decay<float2>(double x) { return (double)expf((float)x); }
-
-template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, double ci, double vr, double vi) {
-  // FP64 argument, reduced mod 2 pi in FP64, then FP32 sincos / exp
-  const double ar = cr * vr - ci * vi, ai = cr * vi + ci * vr;
-  const double twopi = 6.283185307179586476925286766559;
-  const double red = ai - twopi * rint(ai * (1.0 / twopi));
-  float s, c;
-  sincosf((float)red, &s, &c);
-  const float m = expf((float)ar);
-  return make_float2(m * c, m * s);
-}
-
-template <typename C, int L, bool CONTIG, int KIND>
-__global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_pass a) {
-  using Rl = typename real_of<C>::type;
-  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, NT = Geom<C, L>::NT, P = Geom<C, L>::P;
-  constexpr int LDS_N = CONTIG ? Geom<C, L>::LDS_CONTIG : Geom<C, L>::LDS_STRIDED;
-  constexpr unsigned CS = sizeof(C);
-  __shared__ Rl lds[LDS_N];
-  __shared__ double red_lds[NT / 64][HGRID_RED_NCHAN];
-
-  const int tid = threadIdx.x;
-  int q, t;
-  if constexpr (CONTIG) {
-    t = tid % TPP;
-    q = tid / TPP;
-  } else {
-    q = tid % P;
-    t = tid / P;
-  }
-  const long n0 = a.n0, n1 = a.n1, n2 = a.n2;
-  const long ntot = n0 * n1 * n2;
-  const long npencils = ntot / L;
-  const long r = (long)blockIdx.x * P + q;
-  const bool active = r < npencils;
-  long base, c1, c2;
-  unsigned astride;
-  if constexpr (CONTIG) {
-    base = r * L;
-    astride = 1;
-    c1 = r / n1;
-    c2 = r - c1 * n1;
-  } else {
-    if (a.axis == 1) {
-      const long o = r / n2, z = r - o * n2;
-      base = o * n1 * n2 + z;
-      astride = (unsigned)n2;
-      c1 = o;
-      c2 = z;
-    } else {
-      base = r;
-      astride = (unsigned)(n1 * n2);
-      c1 = r / n2;
-      c2 = r - c1 * n2;
-    }
-  }
-  if (!active) {
-    base = 0;
-    c1 = c2 = 0;
-  }
-  // element j of this thread: flat index base + (t + TPP*j)*astride
-  const unsigned e0 = (unsigned)base + (unsigned)t * astride;  // flat index of element 0
-  const unsigned estep = (unsigned)TPP * astride;              // flat index step per j
-  const unsigned fl = a.flags;
-  const C *__restrict__ tw = (const C *)a.tw;
-  const auto rin = make_rsrc(a.in, (unsigned long)ntot * CS);
-
-  C v[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = active ? bload<C>(rin, e0 * CS, (unsigned)j * estep * CS) : mk<C>(0, 0);
-
-  double acc[HGRID_RED_NCHAN] = {0.0, 0.0, 0.0, 0.0, 0.0};
-
-  if constexpr (KIND == KIND_FFT) {
-    if (fl & HGRID_PF_PRESCALE) {
-      const C s = mk<C>((Rl)a.scale[0], (Rl)a.scale[1]);
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = cmul(v[j], s);
-    }
-    if (fl & (HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF)) {
-      const auto rrho = make_rsrc(a.rho_out, (unsigned long)ntot * 8);
-      const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const unsigned e = e0 + (unsigned)j * estep;
-        const double d = cnorm2(v[j]);
-        const double vx = (a.vext && active) ? bload_f64(rvx, e * 8u, 0) : 0.0;
-        if (active) {
-          if (fl & HGRID_PF_PRE_VEFF) bstore_f64(vx + a.g * d, rrho, e * 8u, 0);
-          else if (fl & HGRID_PF_PRE_RHO) bstore_f64(d, rrho, e * 8u, 0);
-        }
-        if (fl & HGRID_PF_PRE_RED) {
-          acc[HGRID_RED_NORM] += d;
-          acc[HGRID_RED_QUART] += d * d;
-          acc[HGRID_RED_VEXT] += vx * d;
-        }
-      }
-    }
-  }
-
-  fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
-
-  if constexpr (KIND == KIND_FFT || KIND == KIND_KIN) {
-    if (fl & HGRID_PF_MID_KRED) {
-      const double kk = a.k2_1[c1] + a.k2_2[c2];
-#pragma unroll
-      for (int j = 0; j < R; ++j) acc[HGRID_RED_KIN] += (kk + a.k2_a[t + TPP * j]) * cnorm2(v[j]);
-    }
-  }
-  if constexpr (KIND == KIND_KIN) {
-    const C *ka = (const C *)a.ktab_a;
-    const C f = cmul(((const C *)a.ktab_1)[c1], ((const C *)a.ktab_2)[c2]);
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cmul(v[j], cmul(ka[t + TPP * j], f));
-  } else if constexpr (KIND == KIND_POT || KIND == KIND_GPE) {
-    const Rl mid = (Rl)a.mid;
-    const auto rpot = make_rsrc(a.pot, (unsigned long)ntot * CS);
-    const auto rvx = make_rsrc(a.vext, (unsigned long)ntot * 8);
-    const auto rrho = make_rsrc(a.rho, (unsigned long)ntot * 8);
-    if constexpr (KIND == KIND_POT) {
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const C pv = bload<C>(rpot, (e0 + (unsigned)j * estep) * CS, 0);
-        v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, a.cons + (double)pv.x, (double)pv.y));
-      }
-    } else {
-      // real potential: V = vext + g rho, or rho alone when it already holds V_eff
-      double vr[R];
-      if (a.vext) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-          const unsigned e = e0 + (unsigned)j * estep;
-          vr[j] = a.cons + bload_f64(rvx, e * 8u, 0) + a.g * bload_f64(rrho, e * 8u, 0);
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < R; ++j) vr[j] = a.cons + bload_f64(rrho, (e0 + (unsigned)j * estep) * 8u, 0);
-      }
-      if (a.c_im == 0.0) {
-        // imaginary time: a real decay factor exp(c_re * V)
-#pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = cscale(v[j], (Rl)((double)mid * real_decay<C>(a.c_re * vr[j])));
-      } else {
-#pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, vr[j], 0.0));
-      }
-    }
-  }
-
-  if constexpr (KIND != KIND_FFT) fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
-
-  if constexpr (KIND == KIND_FFT) {
-    if (a.post != 1.0) {
-      const Rl post = (Rl)a.post;
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = cscale(v[j], post);
-    }
-    if (fl & HGRID_PF_POST_RED) {
-#pragma unroll
-      for (int j = 0; j < R; ++j) acc[HGRID_RED_POST] += cnorm2(v[j]);
-    }
-  }
-  if (!(fl & HGRID_PF_NOSTORE) && active) {
-    const auto rout = make_rsrc(a.out, (unsigned long)ntot * CS);
-#pragma unroll
-    for (int j = 0; j < R; ++j) bstore<C>(v[j], rout, e0 * CS, (unsigned)j * estep * CS);
-  }
-
-  if (fl & (HGRID_PF_PRE_RED | HGRID_PF_MID_KRED | HGRID_PF_POST_RED)) {
-    // deterministic workgroup reduction: wave butterflies, then waves in order
-#pragma unroll
-    for (int c = 0; c < HGRID_RED_NCHAN; ++c) {
-      double x = acc[c];
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-      acc[c] = x;
-    }
-    const int wave = tid / 64, lane = tid % 64;
-    if (lane == 0) {
-#pragma unroll
-      for (int c = 0; c < HGRID_RED_NCHAN; ++c) red_lds[wave][c] = acc[c];
-    }
-    __syncthreads();
-    if (tid < HGRID_RED_NCHAN) {
-      double s = 0.0;
-#pragma unroll
-      for (int w = 0; w < NT / 64; ++w) s += red_lds[w][tid];
-      a.partials[(long)tid * gridDim.x + blockIdx.x] = s;
-    }
-  }
-}
-
-template <typename C, int L, bool CONTIG, int KIND> int launch(const hgrid_pass *p) {
-  constexpr int NT = Geom<C, L>::NT, P = Geom<C, L>::P;
-  const long npencils = (p->n0 * p->n1 * p->n2) / L;
-  const long nblocks = (npencils + P - 1) / P;
-  if (nblocks <= 0) return 0;
-  hipLaunchKernelGGL((fft_pass_kernel<C, L, CONTIG, KIND>), dim3((unsigned)nblocks), dim3(NT), 0, stream(), *p);
-  return check_launch("fft_pass");
-}
-
-template <typename C, bool CONTIG, int KIND> int launch_len(const hgrid_pass *p, long L) {
-  switch (L) {
-    case 2: return launch<C, 2, CONTIG, KIND>(p);
-    case 4: return launch<C, 4, CONTIG, KIND>(p);
-    case 8: return launch<C, 8, CONTIG, KIND>(p);
-    case 16: return launch<C, 16, CONTIG, KIND>(p);
-    case 32: return launch<C, 32, CONTIG, KIND>(p);
-    case 64: return launch<C, 64, CONTIG, KIND>(p);
-    case 128: return launch<C, 128, CONTIG, KIND>(p);
-    case 256: return launch<C, 256, CONTIG, KIND>(p);
-    case 512: return launch<C, 512, CONTIG, KIND>(p);
-    case 1024: return launch<C, 1024, CONTIG, KIND>(p);
-    default: return -1;
-  }
-}
-
-template <typename C> int launch_prec(const hgrid_pass *p, long L, int kind) {
-  const bool contig = p->axis == 2;
-  switch (kind) {
-    case KIND_FFT: return contig ? launch_len<C, true, KIND_FFT>(p, L) : launch_len<C, false, KIND_FFT>(p, L);
-    case KIND_KIN: return contig ? launch_len<C, true, KIND_KIN>(p, L) : launch_len<C, false, KIND_KIN>(p, L);
-    case KIND_POT: return contig ? launch_len<C, true, KIND_POT>(p, L) : -1;
-    case KIND_GPE: return contig ? launch_len<C, true, KIND_GPE>(p, L) : -1;
-    default: return -1;
-  }
-}
-
-template <typename C> long pencils_per_block(long L) {
-  switch (L) {
-    case 2: return Geom<C, 2>::P;
-    case 4: return Geom<C, 4>::P;
-    case 8: return Geom<C, 8>::P;
-    case 16: return Geom<C, 16>::P;
-    case 32: return Geom<C, 32>::P;
-    case 64: return Geom<C, 64>::P;
-    case 128: return Geom<C, 128>::P;
-    case 256: return Geom<C, 256>::P;
-    case 512: return Geom<C, 512>::P;
-    case 1024: return Geom<C, 1024>::P;
+extern "C" int hgrid_fft_supported(long n) {
+  if (n >= 2 && n <= 1024 && (n & (n - 1)) == 0) return 1;
+  switch (n) {
+    case 3: case 6: case 12: case 24: case 48: case 96: case 192: case 384: return 1;
     default: return 0;
   }
 }
-
-}  // namespace lg
-
-extern "C" int hgrid_fft_supported(long n) { return n >= 2 && n <= 1024 && (n & (n - 1)) == 0; }
 
 static long axis_len(long n0, long n1, long n2, int axis) { return axis == 0 ? n0 : axis == 1 ? n1 : n2; }
 
@@ -442,7 +34,7 @@ extern "C" int hgrid_fft_pass(const hgrid_pass *p) {
   else kind = lg::KIND_FFT;
   if (kind == lg::KIND_FFT && p->nfft != 1) return -1;
   if (kind != lg::KIND_FFT && p->nfft != 2) return -1;
-  if (p->prec == HGRID_C128) return lg::launch_prec<double2>(p, L, kind);
-  if (p->prec == HGRID_C64) return lg::launch_prec<float2>(p, L, kind);
+  if (p->prec == HGRID_C128) return lg::launch_f64(p, L, kind);
+  if (p->prec == HGRID_C64) return lg::launch_f32(p, L, kind);
   return -1;
 }

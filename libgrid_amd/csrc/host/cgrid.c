@@ -409,7 +409,7 @@ static void check_fft_dims(cgrid *g) {
   INT d[3] = {g->nx, g->ny, g->nz};
   for (int a = 0; a < 3; ++a)
     if (d[a] > 1 && !hgrid_fft_supported(d[a]))
-      lg_fatal("FFT length %ld is not supported (powers of two up to 1024)", (long)d[a]);
+      lg_fatal("FFT length %ld is not supported (powers of two up to 1024, 3*2^k up to 384)", (long)d[a]);
 }
 
 void cgrid_fft(cgrid *grid) {

@@ -12,6 +12,7 @@
  * and the 1/N of both normalised inverse transforms is folded into the kinetic tables.
  */
 #include "internal.h"
+#include "relax_rule.h"
 
 char grid_analyze_method = (char)-1; /* 0 = finite difference, 1 = FFT, -1 = not set */
 
@@ -345,7 +346,7 @@ static void check_fft_grid(cgrid *g) {
   INT d[3] = {g->nx, g->ny, g->nz};
   for (int a = 0; a < 3; ++a)
     if (d[a] > 1 && !hgrid_fft_supported(d[a]))
-      lg_fatal("FFT length %ld is not supported (powers of two up to 1024)", (long)d[a]);
+      lg_fatal("FFT length %ld is not supported (powers of two up to 1024, 3*2^k up to 384)", (long)d[a]);
 }
 
 void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
@@ -539,8 +540,10 @@ INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
   lg_ensure_device();
   lg_hip(hgrid_malloc((void **)&hist, sizeof(double) * HGRID_RED_NCHAN * batch), "hipMalloc (energy history)");
   double host[HGRID_RED_NCHAN * 16];
-  double prev = NAN, e = NAN;
+  double e = NAN;
   INT done = -1, n = 0;
+  lg_relax_state rs;
+  lg_relax_init(&rs);
   while (n < max_steps && done < 0) {
     INT nb = (max_steps - n) < batch ? (max_steps - n) : batch;
     for (INT b = 0; b < nb; ++b) {
@@ -552,11 +555,10 @@ INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
     for (INT b = 0; b < nb; ++b) {
       e = gpe_energy_from_parts(gwf, host + HGRID_RED_NCHAN * b, g_nl);
       /* host[b] is the energy of the state that entered step n + b + 1 */
-      if (!isnan(prev) && fabs(e - prev) < tol) {
+      if (lg_relax_feed(&rs, (long)(n + b), e, tol)) {
         done = n + b + 1;
         break;
       }
-      prev = e;
     }
     n += nb;
   }

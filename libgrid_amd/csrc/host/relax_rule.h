@@ -1,0 +1,68 @@
+/*
+ * Convergence rule of imaginary-time relaxation, shared by the FP64 relaxation on the host
+ * (grid_wf.c) and the batched FP32 kernel (hip/batch2d.hpp, which defines LG_RELAX_FN as a
+ * device function qualifier before including this file).
+ *
+ * Imaginary time converges geometrically, E_n - E_inf ~ R rho^n.  The energies are sampled at
+ * checkpoints of stride m; with the decreases D1 = E(n-2m) - E(n-m) and D2 = E(n-m) - E(n) the
+ * ratio q = D2 / D1 estimates rho^m and E(n) - E_inf ~ D2 q / (1 - q).  The relaxation stops
+ * when D2 < tol and that remaining error is below tol.  While q > 1/2 the stride doubles (up
+ * to LG_RELAX_MAX_STRIDE), so a window always spans about one decay time and its decrease is
+ * of the order of the remaining error rather than of the rounding noise of a single step (FP32
+ * energies scatter by a few 1e-9 per step, which makes one-step ratios meaningless near
+ * 1e-8).  A decrease D2 that is negative and smaller than tol means rounding dominates the
+ * energy change: that also ends the relaxation.
+ */
+#ifndef LG_RELAX_RULE_H
+#define LG_RELAX_RULE_H
+
+#ifndef LG_RELAX_FN
+#define LG_RELAX_FN static inline
+#endif
+
+#define LG_RELAX_MIN_STRIDE 4
+#define LG_RELAX_MAX_STRIDE 4096
+
+typedef struct {
+  double c0, c1; /* energies at the two previous checkpoints */
+  long m;        /* stride */
+  long next;     /* step of the next checkpoint */
+  int nchk;      /* checkpoints taken at this stride (0, 1, 2) */
+} lg_relax_state;
+
+LG_RELAX_FN void lg_relax_init(lg_relax_state *s) {
+  s->c0 = s->c1 = 0.0;
+  s->m = LG_RELAX_MIN_STRIDE;
+  s->next = 0;
+  s->nchk = 0;
+}
+
+/* Feed E_n, the energy of the state entering step n (n = 0, 1, 2, ...); returns 1 once the
+ * energy is converged to tol. */
+LG_RELAX_FN int lg_relax_feed(lg_relax_state *s, long n, double e, double tol) {
+  if (n != s->next) return 0;
+  s->next = n + s->m;
+  if (s->nchk < 2) {
+    if (s->nchk == 0) s->c0 = e;
+    else s->c1 = e;
+    ++s->nchk;
+    return 0;
+  }
+  const double d1 = s->c0 - s->c1, d2 = s->c1 - e;
+  if (d2 <= 0.0 && -d2 < tol) return 1;
+  const int geometric = d1 > 0.0 && d2 > 0.0 && d2 < d1;
+  const double q = geometric ? d2 / d1 : 1.0;
+  if (geometric && d2 < tol && d2 * q < tol * (1.0 - q)) return 1;
+  if (d1 > 0.0 && d2 > 0.0 && q > 0.5 && s->m < LG_RELAX_MAX_STRIDE) {
+    /* double the stride: the checkpoints E(n - 2m) and E(n) are the new pair */
+    s->c1 = e;
+    s->m *= 2;
+    s->next = n + s->m;
+    return 0;
+  }
+  s->c0 = s->c1;
+  s->c1 = e;
+  return 0;
+}
+
+#endif

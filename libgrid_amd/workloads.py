@@ -15,6 +15,11 @@ Workloads:
                        about an axis drawn uniformly on the sphere (core profile tanh(d / xi),
                        xi = 1, phase winding atan2(axial offset, radial offset - R)); real time
                        dt = 0.02, 500 steps.
+  wells_2d             256 independent 2-D problems (1 x n x n, n uniform in {64, 96, 128, 192, 256},
+                       step 0.2): V = -sum_w depth_w exp(-|r - c_w|^2 / (2 width_w^2)) with 3..8
+                       wells, depth U[0.05, 0.5], width U[0.5, 3], centres uniform in the middle 60%
+                       of the box; start state a Gaussian of standard deviation 0.15 * box at the
+                       centre; imaginary-time relaxation.
 """
 from __future__ import annotations
 
@@ -106,3 +111,27 @@ def vortex_ring(seed: int = 0, n: int = 512, step: float = 0.15, sigma: float = 
     params = dict(shape=shape, step=step, mass=1.0, g=g, omega=omega, radius=radius, axis=axis.tolist(), sigma=sigma,
                   xi=xi, dt=0.02, steps=500)
     return psi, vext, params
+
+
+def wells_2d(seed: int = 0, count: int = 256, step: float = 0.2, sizes=(64, 96, 128, 192, 256)):
+    """List of (psi, v, params) for the batched 2-D ground-state solves."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(sizes))
+        box = n * step
+        nw = int(rng.integers(3, 9))
+        depth = rng.uniform(0.05, 0.5, nw)
+        width = rng.uniform(0.5, 3.0, nw)
+        centre = rng.uniform(-0.3 * box, 0.3 * box, (nw, 2))
+        y = axes(n, step)[:, None]
+        z = axes(n, step)[None, :]
+        v = np.zeros((n, n))
+        for d, wd, c in zip(depth, width, centre):
+            v -= d * np.exp(-((y - c[0]) ** 2 + (z - c[1]) ** 2) / (2 * wd * wd))
+        sig = 0.15 * box
+        psi = np.exp(-(y * y + z * z) / (2 * sig * sig)).astype(np.complex128)
+        psi = normalize(psi.reshape(1, n, n), step)
+        out.append((psi, v.reshape(1, n, n), dict(n=n, step=step, wells=nw, depth=depth.tolist(), width=width.tolist(),
+                                                 centre=centre.tolist())))
+    return out

@@ -1,0 +1,89 @@
+"""The imaginary-time convergence rule (libgrid_amd/csrc/host/relax_rule.h), compiled with gcc
+into a small harness and fed synthetic energy sequences: geometric tails of known limit, with
+and without FP32-sized rounding noise.  The rule must stop only once the true remaining error
+E_n - E_inf is of the order of tol, including for slow (nearly degenerate) decays."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HDR = os.path.join(HERE, "..", "libgrid_amd", "csrc", "host")
+
+SRC = r"""
+#include "relax_rule.h"
+long run(const double *e, long n, double tol) {
+  lg_relax_state s;
+  lg_relax_init(&s);
+  for (long k = 0; k < n; ++k)
+    if (lg_relax_feed(&s, k, e[k], tol)) return k;
+  return -1;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def rule(tmp_path_factory):
+    d = tmp_path_factory.mktemp("relax")
+    c = d / "r.c"
+    c.write_text(SRC)
+    so = d / "r.so"
+    subprocess.check_call(["gcc", "-O2", "-shared", "-fPIC", "-I", HDR, str(c), "-o", str(so), "-lm"])
+    lib = C.CDLL(str(so))
+    lib.run.restype = C.c_long
+    lib.run.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_double]
+
+    def stop(e, tol):
+        a = np.ascontiguousarray(e, dtype=np.float64)
+        return lib.run(a.ctypes.data_as(C.POINTER(C.c_double)), len(a), tol)
+
+    return stop
+
+
+def tail(rho, n, r0=1e-2, einf=-0.3):
+    k = np.arange(n)
+    return einf + r0 * rho ** k, einf
+
+
+@pytest.mark.parametrize("rho", [0.5, 0.9, 0.97, 0.995, 0.9996])
+def test_geometric_exact(rule, rho):
+    tol = 1e-10
+    e, einf = tail(rho, 400000)
+    k = rule(e, tol)
+    assert k > 0
+    rem = e[k] - einf
+    # stops with the remaining error below tol, and not needlessly late (within 4x of tol or
+    # within one stride of the first step where it holds)
+    assert rem < tol
+    first = int(np.argmax(e - einf < tol))
+    assert rem > tol / 64 or k - first <= max(64, int(2.0 / (1 - rho)))
+
+
+@pytest.mark.parametrize("rho", [0.97, 0.995, 0.9996])
+def test_geometric_fp32_noise(rule, rho):
+    """Energies with 5e-9 Gaussian scatter per step (what FP32 states give): a one-step test
+    stops orders of magnitude early on slow tails; the windowed rule stays within a few tol."""
+    tol = 1e-8
+    rng = np.random.default_rng(1)
+    clean, einf = tail(rho, 200000)
+    worst = 0.0
+    for trial in range(20):
+        k = rule(clean + rng.normal(0.0, 5e-9, clean.shape), tol)
+        assert k > 0
+        worst = max(worst, clean[k] - einf)  # true remaining error where it stopped
+    print(f"rho {rho}: worst remaining error {worst:.2e}")
+    assert worst < 5 * tol
+
+
+def test_never_stops_with_zero_tol(rule):
+    e, _ = tail(0.9, 5000)
+    assert rule(e, 0.0) == -1
+
+
+def test_rise_below_tol_ends(rule):
+    # a flat sequence that then rises by less than tol: rounding floor
+    e = np.full(100, -0.5)
+    e[40:] += 1e-12
+    assert 0 < rule(e, 1e-10) < 100
