@@ -281,4 +281,34 @@ INT grid_batch2d_history(INT solve, REAL *energy, INT len);
 void grid_batch2d_set_tile(int variant, int nthreads, int per_cu);
 void grid_batch2d_get_tile(int *variant, int *nthreads, int *per_cu);
 
+/* ---- learned warm start (this build's addition; no counterpart in the reference) --------
+ * A 3-D convolutional U-Net (about 1.45 M parameters, BF16 weights and activations, FP32
+ * accumulation on the matrix cores) maps a potential sampled on an S^3 grid to a ground-state
+ * amplitude, which grid_wf_warm_start interpolates onto the wavefunction's n^3 grid (n a
+ * multiple of S) as the start of imaginary-time relaxation.  The relaxation alone decides the
+ * converged energy; with no network the plain Gaussian start is used. */
+typedef struct grid_surrogate grid_surrogate;
+/* side S: power of two >= 16; batch: samples per forward/backward; vscale: input scale of V */
+grid_surrogate *grid_surrogate_alloc(INT side, INT batch, REAL vscale, INT seed);
+void grid_surrogate_free(grid_surrogate *s);
+INT grid_surrogate_nparams(grid_surrogate *s);
+INT grid_surrogate_side(grid_surrogate *s);
+/* parameters in layer order e1 e2a e2b e3a e3b ba bb d3a d3b d2a d2b d1 (W [Cout][27][Cin]
+ * then b [Cout] each), then the head (16 weights, 1 bias); setting them resets Adam */
+void grid_surrogate_set_params(grid_surrogate *s, const REAL *p);
+void grid_surrogate_get_params(grid_surrogate *s, REAL *p);
+/* loss (mean squared error) and its gradient (may be NULL) on nb <= batch samples [nb][S^3] */
+REAL grid_surrogate_gradient(grid_surrogate *s, INT nb, const REAL *pot, const REAL *target, REAL *grad);
+REAL grid_surrogate_loss(grid_surrogate *s, INT n, const REAL *pot, const REAL *target);
+void grid_surrogate_predict(grid_surrogate *s, INT n, const REAL *pot, REAL *amp);
+/* copy n samples to the device, then train for epochs (Adam, cosine-decayed lr) */
+void grid_surrogate_set_training_data(grid_surrogate *s, INT n, const REAL *pot, const REAL *target);
+REAL grid_surrogate_train(grid_surrogate *s, INT epochs, REAL lr, INT seed, REAL *loss_history);
+int grid_surrogate_save(grid_surrogate *s, const char *path);
+grid_surrogate *grid_surrogate_load(const char *path, INT batch);
+/* starts: the network's (s may be NULL: Gaussian of fallback_width) and the plain Gaussian
+ * exp(-r^2 / (2 width^2)) at the grid centre; both normalised */
+void grid_wf_warm_start(wf *gwf, rgrid *potential, grid_surrogate *s, REAL fallback_width);
+void grid_wf_gaussian_start(wf *gwf, REAL width);
+
 #endif

@@ -215,6 +215,52 @@ int hgrid_b2d_cus(void);            /* compute units of the device */
  * solves from the queue until it is empty). */
 int hgrid_b2d_launch(const hgrid_b2d_job *jobs, int njobs, int *queue, int variant, int nthreads, int nblocks);
 
+/* ---- warm-start network kernels (BF16 activations, FP32 accumulation) ---------------------
+ * No counterpart in the reference.  Activations are channels-last [voxel][channel] BF16 with a
+ * channel stride (s) and offset (o) per tensor, so concatenations are slices of one buffer.
+ * Voxels m = ((b S + z) S + y) S + x over cubes of side S = 2^lgS; 3x3x3 convolutions wrap
+ * periodically within each sample.  Channel counts are multiples of 8 (16 for outputs). */
+
+/* y[m][yo + co] = act(sum_t sum_ci x[m + d_t][xo + ci] w[co][t][ci] + bias[co]) (bias may be
+ * NULL; act = ReLU if relu) and then zero where mask[m][mko + co] <= 0 (mask may be NULL).
+ * With wt from hgrid_nn_wprep in place of w it is the data gradient of the convolution. */
+int hgrid_nn_conv3(const void *x, int xs, int xo, int cin, const void *w, const float *bias, void *y, int ys, int yo,
+                   int cout, const void *mask, int mks, int mko, int lgS, long nvox, int relu);
+/* partial[p][co][t][ci] of dW = sum_m dy[m][dyo + co] x[m + d_t][xo + ci]; sum the
+ * hgrid_nn_wgrad_partials(nvox) partials with hgrid_nn_reduce. */
+int hgrid_nn_conv3_wgrad(const void *dy, int dys, int dyo, int cout, const void *x, int xs, int xo, int cin, int lgS,
+                         long nvox, float *partial);
+long hgrid_nn_wgrad_partials(long nvox);
+long hgrid_nn_wgrad_chunk(long nvox);
+int hgrid_nn_reduce(const float *partial, long np, long n, float *out, int accumulate);
+int hgrid_nn_reduce_strided(const float *partial, long np, int stride, int ncols, float *out);
+/* partial[chunk][c] = column sums of d[.][dof + c]; hgrid_nn_colsum_chunks(nvox) chunks */
+int hgrid_nn_colsum(const void *d, int ds, int dof, int c, long nvox, float *partial);
+long hgrid_nn_colsum_chunks(long nvox);
+/* nearest upsampling to the fine level (side 2^lgF) and its gradient times (act > 0) */
+int hgrid_nn_up2(const void *src, int ss, int so, int c, void *dst, int ds, int dof, int lgF, long nfine);
+int hgrid_nn_up2_bwd(const void *dfine, int fs, int fo, int c, const void *act, int as, int ao, void *dst, int lgC,
+                     long ncoarse);
+/* 2x2x2 average pooling to a dense coarse tensor, and the gradient of a pooled activation plus
+ * its skip gradient (dskip may be NULL) times (act > 0), written dense */
+int hgrid_nn_pool2(const void *src, int ss, int so, int c, void *dst, int lgC, long ncoarse);
+int hgrid_nn_pool2_bwd(const void *dcoarse, const void *dskip, int ks, int ko, int c, const void *act, int as, int ao,
+                       void *dst, int lgF, long nfine);
+/* 1x1 head over 16 channels (wh: 16 weights then the bias) and its backward for the loss
+ * sum (out - target)^2 * scale / 2: writes the masked input gradient and per-block partials
+ * [block][18] (16 weight gradients, bias gradient, squared error) */
+int hgrid_nn_head_fwd(const void *d, const float *wh, float *out, long nvox);
+int hgrid_nn_head_bwd(const void *d, const float *wh, const float *out, const float *target, float scale, void *dpre,
+                      long nvox, float *partial);
+long hgrid_nn_head_blocks(long nvox);
+/* BF16 weights wb[co][t][ci] and reflected-transposed wt[ci][26 - t][co] from FP32 w */
+int hgrid_nn_wprep(const float *w, int cout, int cin, void *wb, void *wt);
+int hgrid_nn_adam(float *w, const float *g, float *m1, float *m2, long n, float lr, float b1, float b2, float eps, int t);
+/* input tensor [m][8]: vscale * v[m], then zeros */
+int hgrid_nn_input(const float *v, float vscale, void *x0, long nvox);
+/* periodic trilinear interpolation of an S^3 field onto n^3 (n = fct S) as a real wavefunction */
+int hgrid_nn_to_grid(const float *a, int lgS, int fct, long n, void *psi, int prec);
+
 #ifdef __cplusplus
 }
 #endif

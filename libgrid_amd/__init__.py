@@ -266,6 +266,21 @@ PROTOTYPES = {
     "grid_wf_batch2d_relax": (D, [L, C.POINTER(WF), C.POINTER(RG), D, D, L, C.POINTER(D), C.POINTER(L)]),
     "grid_batch2d_autotune": (C.c_char_p, [L, C.POINTER(WF), C.POINTER(RG), D, L]),
     "grid_batch2d_solve_times": (L, [C.POINTER(D), C.POINTER(D)]),
+    "grid_surrogate_alloc": (C.c_void_p, [L, L, D, L]),
+    "grid_surrogate_free": (V, [C.c_void_p]),
+    "grid_surrogate_nparams": (L, [C.c_void_p]),
+    "grid_surrogate_side": (L, [C.c_void_p]),
+    "grid_surrogate_set_params": (V, [C.c_void_p, C.POINTER(D)]),
+    "grid_surrogate_get_params": (V, [C.c_void_p, C.POINTER(D)]),
+    "grid_surrogate_gradient": (D, [C.c_void_p, L, C.POINTER(D), C.POINTER(D), C.POINTER(D)]),
+    "grid_surrogate_loss": (D, [C.c_void_p, L, C.POINTER(D), C.POINTER(D)]),
+    "grid_surrogate_predict": (V, [C.c_void_p, L, C.POINTER(D), C.POINTER(D)]),
+    "grid_surrogate_set_training_data": (V, [C.c_void_p, L, C.POINTER(D), C.POINTER(D)]),
+    "grid_surrogate_train": (D, [C.c_void_p, L, D, L, C.POINTER(D)]),
+    "grid_surrogate_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "grid_surrogate_load": (C.c_void_p, [C.c_char_p, L]),
+    "grid_wf_warm_start": (V, [WF, RG, C.c_void_p, D]),
+    "grid_wf_gaussian_start": (V, [WF, D]),
     "grid_batch2d_record_history": (V, [L]),
     "grid_batch2d_history": (L, [L, C.POINTER(D), L]),
     "grid_batch2d_set_tile": (V, [C.c_int, C.c_int, C.c_int]),

@@ -27,6 +27,11 @@ int check(hipError_t e, const char *what) {
 }
 
 int check_launch(const char *what) { return check(hipGetLastError(), what); }
+
+int fail(const char *what) {
+  std::snprintf(g_err, sizeof(g_err), "%s", what);
+  return -1;
+}
 }  // namespace lg
 
 using lg::check;

@@ -7,4 +7,6 @@ hipStream_t stream();
 // Record a HIP error (if any) from the last call / launch; returns 0 or -1.
 int check(hipError_t e, const char *what);
 int check_launch(const char *what);
+// Record an argument error (nothing was launched); returns -1.
+int fail(const char *what);
 }  // namespace lg

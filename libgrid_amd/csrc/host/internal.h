@@ -69,6 +69,8 @@ const double *lg_r_in(rgrid *g);
 double *lg_r_out(rgrid *g);
 void lg_r_written(rgrid *g);
 double lg_step_factor(INT nx, INT ny, INT nz, REAL step); /* step^d for dims != 1 */
+unsigned long long lg_splitmix64(unsigned long long *state); /* seeded stream (SplitMix64) */
+double lg_normal(unsigned long long *state);                 /* N(0, 1) by Box-Muller */
 
 /* planner (fftplan.c) */
 typedef struct lg_shape {

@@ -130,6 +130,22 @@ const void *lg_twiddles(long n, int prec) {
   return d;
 }
 
+/* ---- seeded random streams --------------------------------------------------------------- */
+
+unsigned long long lg_splitmix64(unsigned long long *state) {
+  unsigned long long z = (*state += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+double lg_normal(unsigned long long *state) {
+  /* two 53-bit uniforms in (0, 1] and (0, 1) */
+  const double u1 = ((double)(lg_splitmix64(state) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+  const double u2 = (double)(lg_splitmix64(state) >> 11) * (1.0 / 9007199254740992.0);
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
 /* ---- threads and timers (reference src/threads.c:89-157, src/ctimer.c:23-60) -------------- */
 
 void grid_threads_init(INT threads) {
