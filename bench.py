@@ -174,6 +174,25 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
     return out
 
 
+def warm_start_bench(lib, count):
+    """Wall time to a 1e-10 Eh ground state on held-out 256^3 wells potentials: plain Gaussian
+    start with FP64 throughout, against the trained network's start with FP32 relaxation
+    (to 1e-8) then FP64 (to 1e-10).  Both timings include building the start state."""
+    from libgrid_amd import warmstart as WS
+
+    if not os.path.exists(WS.WEIGHTS):
+        log(f"warm start: no trained network at {WS.WEIGHTS}; skipped")
+        return None
+    net = lib.grid_surrogate_load(WS.WEIGHTS.encode(), 1)
+    modes = ("gauss64", "net32", "net64")
+    recs = WS.evaluate(lib, net, count, WS.FINE, modes)
+    lib.grid_surrogate_free(net)
+    summ = WS.summarize(recs, modes)
+    log(f"warm start: {json.dumps(summ)}")
+    return dict(summary=summ, potentials=[r["index"] for r in recs], seed=WS.HELDOUT_SEED,
+                grid=[WS.FINE] * 3, box_bohr=WS.BOX, dt=WS.DT, tol_Eh=1e-10)
+
+
 def cpu_baseline(psi, vext, prm, steps):
     from oracle import oracle as orc
 
@@ -257,6 +276,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gs", action="store_true", help="skip the time-to-ground-state runs")
     ap.add_argument("--gs-max-steps", type=int, default=20000)
+    ap.add_argument("--ws-count", type=int, default=2,
+                    help="held-out 256^3 wells potentials for the warm-start time-to-ground-state comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -314,6 +335,10 @@ def main():
     if not args.no_gs:
         gs = ground_state(lib, LG, psi, vext, prm, n, args.gs_max_steps)
 
+    ws = None
+    if not args.no_gs and args.ws_count > 0:
+        ws = warm_start_bench(lib, args.ws_count)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -348,6 +373,7 @@ def main():
         "copy_ceiling_GBps": copy["gbps"],
         "energy": energy,
         "time_to_ground_state": gs,
+        "warm_start": ws,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -69,7 +69,7 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
 #define HGRID_PF_PRESCALE   0x0001u  /* multiply loads by the complex device scalar *scale */
 #define HGRID_PF_PRE_RHO    0x0002u  /* pre: rho_out = |psi|^2 (GPE density of the input state) */
 #define HGRID_PF_PRE_RED    0x0004u  /* pre: partial sums of |psi|^2, vext|psi|^2, |psi|^4 */
-#define HGRID_PF_MID_KIN    0x0008u  /* mid: psi *= ktab_a[n] * ktab_1[c1] * ktab_2[c2] */
+#define HGRID_PF_MID_KIN    0x0008u  /* mid: psi *= ktab_a[n] * ktab_1[c1] * ktab_2[c2] (double2 tables) */
 #define HGRID_PF_MID_KRED   0x0010u  /* mid: partial sum of k^2 |psi_k|^2 (before MID_KIN) */
 #define HGRID_PF_MID_POT    0x0020u  /* mid: psi *= mid * exp(c * (cons + pot)), complex pot */
 #define HGRID_PF_MID_GPE    0x0040u  /* mid: psi *= mid * exp(c * (cons + vext + g * rho)), or
@@ -164,7 +164,8 @@ int hgrid_sum_real(double *out, const double *a, long n, int squared, double *sc
 /* psi *= exp(c * (cons + pot)) with c = (c_re, c_im); pot complex (same precision). */
 int hgrid_potential_phase(void *psi, const void *pot, int prec, long n, double c_re, double c_im, double cons,
                           const double *scale);
-/* psi[i,j,k] *= ta[i] * tb[j] * tc[k]  (3 tables of complex factors, precision prec) */
+/* psi[i,j,k] *= ta[i] * tb[j] * tc[k]  (3 tables of double2 factors for either precision;
+ * the factor is formed in FP64 and rounded once onto psi) */
 int hgrid_kinetic_phase(void *psi, int prec, long nx, long ny, long nz, const void *ta, const void *tb,
                         const void *tc, const double *scale);
 /* psi *= (c_re + i c_im) * (*scale if scale) */

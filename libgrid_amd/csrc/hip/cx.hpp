@@ -32,6 +32,18 @@ template <typename C> LG_CX C cneg(C a) { return mk<C>(-a.x, -a.y); }
 template <typename C> LG_CX C cconj(C a) { return mk<C>(a.x, -a.y); }
 template <typename C> LG_CX C cmul(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 template <typename C> LG_CX C cscale(C a, typename real_of<C>::type s) { return mk<C>(a.x * s, a.y * s); }
+// an FP64 complex rounded once to the precision of C
+template <typename C> LG_CX C round_to(double2 k) {
+  using R = typename real_of<C>::type;
+  return mk<C>((R)k.x, (R)k.y);
+}
+// a * k with the product formed in FP64 and rounded once to a's precision (FP64 phase tables
+// applied to FP32 data: the table moduli are not rounded step after step)
+template <typename C> LG_CX C cmul_d(C a, double2 k) {
+  using R = typename real_of<C>::type;
+  const double x = (double)a.x, y = (double)a.y;
+  return mk<C>((R)(x * k.x - y * k.y), (R)(x * k.y + y * k.x));
+}
 // multiply by -i and +i
 template <typename C> LG_CX C mul_mi(C a) { return mk<C>(a.y, -a.x); }
 template <typename C> LG_CX C mul_pi(C a) { return mk<C>(-a.y, a.x); }

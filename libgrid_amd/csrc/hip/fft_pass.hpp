@@ -250,10 +250,11 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     }
   }
   if constexpr (KIND == KIND_KIN) {
-    const C *ka = (const C *)a.ktab_a;
-    const C f = cmul(((const C *)a.ktab_1)[c1], ((const C *)a.ktab_2)[c2]);
+    // FP64 tables whatever the storage precision; the factor is formed in FP64
+    const double2 *ka = (const double2 *)a.ktab_a;
+    const double2 f = cmul(((const double2 *)a.ktab_1)[c1], ((const double2 *)a.ktab_2)[c2]);
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cmul(v[j], cmul(ka[t + TPP * j], f));
+    for (int j = 0; j < R; ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
   } else if constexpr (KIND == KIND_POT || KIND == KIND_GPE) {
     const Rl mid = (Rl)a.mid;
     const auto rpot = make_rsrc(a.pot, (unsigned long)ntot * CS);

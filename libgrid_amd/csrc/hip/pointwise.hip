@@ -270,15 +270,15 @@ __global__ void k_potential_phase(C *psi, const C *pot, long n, double cr, doubl
 }
 
 template <typename C>
-__global__ void k_kinetic_phase(C *psi, long nx, long ny, long nz, const C *ta, const C *tb, const C *tc,
-                                const double *scale) {
+__global__ void k_kinetic_phase(C *psi, long nx, long ny, long nz, const double2 *ta, const double2 *tb,
+                                const double2 *tc, const double *scale) {
   const long n = nx * ny * nz;
-  const C s = scal<C>(scale);
   LG_GRID_STRIDE(idx, n) {
     const long i = idx / (ny * nz), j = (idx / nz) % ny, k = idx % nz;
-    C x = psi[idx];
-    if (scale) x = cmul(x, s);
-    psi[idx] = cmul(x, cmul(ta[i], cmul(tb[j], tc[k])));
+    // FP64 tables (and pending scale) multiplied in FP64, rounded once on the store
+    double2 f = cmul(ta[i], cmul(tb[j], tc[k]));
+    if (scale) f = cmul(f, make_double2(scale[0], scale[1]));
+    psi[idx] = cmul_d(psi[idx], f);
   }
 }
 
@@ -470,8 +470,8 @@ extern "C" int hgrid_kinetic_phase(void *psi, int prec, long nx, long ny, long n
   if (prec == HGRID_C128)
     LG_LAUNCH(k_kinetic_phase<double2>, n, (double2 *)psi, nx, ny, nz, (const double2 *)ta, (const double2 *)tb,
               (const double2 *)tc, scale);
-  LG_LAUNCH(k_kinetic_phase<float2>, n, (float2 *)psi, nx, ny, nz, (const float2 *)ta, (const float2 *)tb,
-            (const float2 *)tc, scale);
+  LG_LAUNCH(k_kinetic_phase<float2>, n, (float2 *)psi, nx, ny, nz, (const double2 *)ta, (const double2 *)tb,
+            (const double2 *)tc, scale);
 }
 
 extern "C" int hgrid_cscale(void *psi, int prec, long n, double cr, double ci, const double *scale) {

@@ -113,18 +113,15 @@ static void cexp_ld(long double ar, long double ai, long double *re, long double
   *im = m * sinl(ai);
 }
 
-static void upload_table(void **dptr, const long double *re, const long double *im, long n, int prec) {
-  size_t es = lg_prec_size(prec);
-  void *h = malloc((size_t)n * es);
+/* Kinetic tables are FP64 (double2) for either storage precision: the kernels form the
+ * factor in FP64 and round once onto FP32 data. */
+static void upload_table(void **dptr, const long double *re, const long double *im, long n) {
+  const size_t es = 16;
+  double *h = (double *)malloc((size_t)n * es);
   if (!h) lg_fatal("out of host memory (tables)");
   for (long i = 0; i < n; ++i) {
-    if (prec == HGRID_C128) {
-      ((double *)h)[2 * i] = (double)re[i];
-      ((double *)h)[2 * i + 1] = (double)im[i];
-    } else {
-      ((float *)h)[2 * i] = (float)re[i];
-      ((float *)h)[2 * i + 1] = (float)im[i];
-    }
+    h[2 * i] = (double)re[i];
+    h[2 * i + 1] = (double)im[i];
   }
   if (!*dptr) lg_hip(hgrid_malloc(dptr, (size_t)n * es), "hipMalloc (tables)");
   lg_hip(hgrid_memcpy_h2d(*dptr, h, (size_t)n * es), "hipMemcpy (tables)");
@@ -178,7 +175,7 @@ ktab_cache *lg_kinetic_tables(cgrid *g, double tm_re, double tm_im, double fold)
         im[i] = xi;
       }
     }
-    upload_table(&k->t[a], re, im, len, d->prec);
+    upload_table(&k->t[a], re, im, len);
     free(re);
     free(im);
   }

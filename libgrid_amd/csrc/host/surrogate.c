@@ -412,17 +412,18 @@ REAL grid_surrogate_loss(grid_surrogate *s, INT n, const REAL *pot, const REAL *
 
 /* ---- persistence ----------------------------------------------------------------------------- */
 
-static const char kMagic[8] = {'L', 'G', 'U', 'N', 'E', 'T', '0', '1'};
+/* file: 8-byte magic, side, parameter count, 0 (int64), vscale (float64), FP32 parameters */
+static const char kMagic[8] = {'L', 'G', 'U', 'N', 'E', 'T', '0', '2'};
 
 int grid_surrogate_save(grid_surrogate *s, const char *path) {
   FILE *fp = fopen(path, "wb");
   if (!fp) return -1;
-  double *h = (double *)malloc(sizeof(double) * s->np);
-  grid_surrogate_get_params(s, h);
+  float *h = (float *)malloc(sizeof(float) * s->np);
+  lg_hip(hgrid_memcpy_d2h(h, s->w, sizeof(float) * s->np), "hipMemcpy (params)");
   const long hdr[3] = {1L << s->lgS, s->np, 0};
   const double vs = s->vscale;
   int ok = fwrite(kMagic, 1, 8, fp) == 8 && fwrite(hdr, sizeof(long), 3, fp) == 3 && fwrite(&vs, sizeof(double), 1, fp) == 1 &&
-           fwrite(h, sizeof(double), s->np, fp) == (size_t)s->np;
+           fwrite(h, sizeof(float), s->np, fp) == (size_t)s->np;
   free(h);
   return (fclose(fp) == 0 && ok) ? 0 : -1;
 }
@@ -444,16 +445,17 @@ grid_surrogate *grid_surrogate_load(const char *path, INT batch) {
     grid_surrogate_free(s);
     return NULL;
   }
-  double *h = (double *)malloc(sizeof(double) * s->np);
-  const size_t got = fread(h, sizeof(double), s->np, fp);
+  float *h = (float *)malloc(sizeof(float) * s->np);
+  const size_t got = fread(h, sizeof(float), s->np, fp);
   fclose(fp);
   if (got != (size_t)s->np) {
     free(h);
     grid_surrogate_free(s);
     return NULL;
   }
-  grid_surrogate_set_params(s, h);
+  lg_hip(hgrid_memcpy_h2d(s->w, h, sizeof(float) * s->np), "hipMemcpy (params)");
   free(h);
+  refresh_weights(s);
   return s;
 }
 

@@ -1,0 +1,243 @@
+"""Learned warm start for imaginary-time ground states: training data from the library's own
+solver, training through the C API, and the comparison with the plain Gaussian start.
+
+Geometry: a periodic box of side BOX bohr; the network works on a COARSE^3 grid (step
+BOX / COARSE), wavefunctions are relaxed on FINE^3 (step BOX / FINE).  Coarse points coincide
+with every (FINE / COARSE)-th fine point, so the network sees the fine potential sampled there.
+Potentials are seeded sums of 4-10 attractive Gaussians (workloads.wells_3d).  Relaxation is
+grid_wf_gpe_relax with g = 0 (linear Schroedinger), time step DT, windowed energy rule; the
+converged solver energy, never the network, decides the result.
+
+Run as a module to build a network:
+    python -m libgrid_amd.warmstart train --out weights.bin [--count 2000 --epochs 30]
+"""
+import argparse
+import ctypes as C
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import WF_2ND_ORDER_FFT, WF_PERIODIC_BOUNDARY, boundary, get_cgrid, load, set_rgrid
+from . import workloads as W
+
+BOX = 32.0
+COARSE = 64
+FINE = 256
+DT = 0.1
+GAUSS_WIDTH = 0.15 * BOX  # plain start: exp(-r^2 / (2 w^2)) at the box centre
+VSCALE = 2.0              # network input: 2 V (V in [-0.5 * wells, 0] Eh)
+TRAIN_SEED = 20240101     # potentials 0 .. count-1 of this stream train; HELDOUT_SEED tests
+HELDOUT_SEED = 777
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet64.bin")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+class Problem:
+    """One wavefunction + potential pair on an n^3 grid of the box."""
+
+    def __init__(self, lg, n, v):
+        self.lg, self.n = lg, n
+        step = BOX / n
+        self.w = lg.grid_wf_alloc(n, n, n, step, 1.0, WF_PERIODIC_BOUNDARY, WF_2ND_ORDER_FFT, b"ws")
+        self.r = lg.rgrid_alloc(n, n, n, step, boundary("rgrid_value_outside_periodic"), None, b"v")
+        set_rgrid(self.r, v)
+
+    def free(self):
+        self.lg.grid_wf_free(self.w)
+        self.lg.rgrid_free(self.r)
+
+    def start(self, net):
+        if net is None:
+            self.lg.grid_wf_gaussian_start(self.w, GAUSS_WIDTH)
+        else:
+            self.lg.grid_wf_warm_start(self.w, self.r, net, GAUSS_WIDTH)
+
+    def relax(self, net, fp32_tol=None, tol=1e-10, max_steps=200000):
+        """Timed: start construction, optional FP32 relaxation to fp32_tol, FP64 to tol."""
+        lg = self.lg
+        g = self.w.contents.grid
+        e = C.c_double()
+        lg.cgrid_set_precision(g, b"\x00")
+        self.start(None)                     # untimed: uploads V, builds tables and plans
+        lg.grid_wf_gpe_energy(self.w, self.r, 0.0)
+        lg.grid_hip_sync()
+        t0 = time.perf_counter()
+        s32 = 0
+        if fp32_tol is not None:
+            lg.cgrid_set_precision(g, b"\x01")
+        self.start(net)
+        if fp32_tol is not None:
+            s32 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, fp32_tol, max_steps, C.byref(e))
+            lg.cgrid_set_precision(g, b"\x00")
+        s64 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, tol, max_steps, C.byref(e))
+        lg.grid_hip_sync()
+        sec = time.perf_counter() - t0
+        energy = lg.grid_wf_gpe_energy(self.w, self.r, 0.0)
+        return dict(seconds=sec, fp32_steps=int(s32), fp64_steps=int(s64), energy=energy,
+                    converged=bool(s64 < max_steps))
+
+
+def potentials(seed, count, n):
+    prm = W.wells_3d_params(seed, count, BOX)
+    return prm, [W.wells_3d(p, n, BOX / n) for p in prm]
+
+
+def ground_state_amplitude(lg, v, tol=1e-7, max_steps=20000):
+    """|psi_0| on the coarse grid from the Gaussian start (FP32 relaxation), scaled to unit
+    mean square; also returns the steps taken."""
+    p = Problem(lg, v.shape[0], v)
+    try:
+        e = C.c_double()
+        lg.cgrid_set_precision(p.w.contents.grid, b"\x01")
+        p.start(None)
+        steps = lg.grid_wf_gpe_relax(p.w, p.r, 0.0, DT, tol, max_steps, C.byref(e))
+        a = np.abs(get_cgrid(p.w.contents.grid))
+        a /= np.sqrt(np.mean(a * a))
+        return a, int(steps), e.value
+    finally:
+        p.free()
+
+
+def make_dataset(lg, seed, count, n=COARSE, tol=1e-7, max_steps=20000, every=100):
+    _, vs = potentials(seed, count, n)
+    V = np.stack(vs)
+    T = np.empty_like(V)
+    steps = np.empty(count, dtype=np.int64)
+    t0 = time.perf_counter()
+    for i, v in enumerate(vs):
+        T[i], steps[i], _ = ground_state_amplitude(lg, v, tol, max_steps)
+        if (i + 1) % every == 0:
+            log(f"  ground states {i + 1}/{count} ({time.perf_counter() - t0:.0f} s, median steps "
+                f"{int(np.median(steps[:i + 1]))})")
+    return V, T, steps
+
+
+def train(lg, V, T, epochs=30, lr=1e-3, batch=8, seed=0):
+    net = lg.grid_surrogate_alloc(V.shape[1], batch, VSCALE, seed)
+    n = V.shape[0]
+    lg.grid_surrogate_set_training_data(net, n, _dp(np.ascontiguousarray(V)), _dp(np.ascontiguousarray(T)))
+    hist = np.zeros(epochs)
+    t0 = time.perf_counter()
+    lg.grid_surrogate_train(net, epochs, lr, seed, _dp(hist))
+    lg.grid_hip_sync()
+    return net, hist, time.perf_counter() - t0
+
+
+def overlap(a, b):
+    return float(np.sum(a * b) / np.sqrt(np.sum(a * a) * np.sum(b * b)))
+
+
+MODES = {  # name: (use the network, FP32 stage tolerance or None)
+    "gauss64": (False, None),
+    "net64": (True, None),
+    "gauss32": (False, 1e-8),
+    "net32": (True, 1e-8),
+}
+
+
+def evaluate(lg, net, count, n=FINE, modes=("gauss64", "net32", "net64"), seed=HELDOUT_SEED, first=0,
+             max_steps=200000, tol=1e-10, every=1):
+    """Relax held-out potentials first .. first+count-1 of the seed's stream on n^3 from each
+    start; returns per-potential records.  The converged energies of all modes must agree."""
+    prm = W.wells_3d_params(seed, first + count, BOX)
+    out = []
+    for i in range(first, first + count):
+        v = W.wells_3d(prm[i], n, BOX / n)
+        p = Problem(lg, n, v)
+        rec = dict(index=i, wells=len(prm[i]["depth"]))
+        try:
+            for m in modes:
+                use, t32 = MODES[m]
+                rec[m] = p.relax(net if use else None, fp32_tol=t32, tol=tol, max_steps=max_steps)
+        finally:
+            p.free()
+        es = [rec[m]["energy"] for m in modes]
+        rec["energy_spread"] = float(max(es) - min(es))
+        out.append(rec)
+        if (len(out) % every) == 0:
+            log(f"  potential {i}: " + "; ".join(
+                f"{m} {rec[m]['seconds']:.3f} s ({rec[m]['fp32_steps']}+{rec[m]['fp64_steps']} steps)" for m in modes)
+                + f"; E {rec[modes[0]]['energy']:.12f}, spread {rec['energy_spread']:.1e}")
+    return out
+
+
+def summarize(recs, modes):
+    s = {}
+    for m in modes:
+        s[m] = dict(seconds_total=float(sum(r[m]["seconds"] for r in recs)),
+                    fp32_steps_mean=float(np.mean([r[m]["fp32_steps"] for r in recs])),
+                    fp64_steps_mean=float(np.mean([r[m]["fp64_steps"] for r in recs])),
+                    converged=int(sum(r[m]["converged"] for r in recs)))
+    base = modes[0]
+    for m in modes[1:]:
+        ratios = [r[base]["seconds"] / r[m]["seconds"] for r in recs]
+        s[m]["speedup_total"] = s[base]["seconds_total"] / s[m]["seconds_total"]
+        s[m]["speedup_median"] = float(np.median(ratios))
+    s["energy_spread_max"] = float(max(r["energy_spread"] for r in recs))
+    s["count"] = len(recs)
+    return s
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", required=True)
+    t.add_argument("--count", type=int, default=2000)
+    t.add_argument("--heldout", type=int, default=200)
+    t.add_argument("--epochs", type=int, default=30)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--batch", type=int, default=8)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--weights", required=True)
+    e.add_argument("--count", type=int, default=200)
+    e.add_argument("--first", type=int, default=0)
+    e.add_argument("--n", type=int, default=FINE)
+    e.add_argument("--modes", default="gauss64,net32,net64,gauss32")
+    e.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    lg = load()
+    if a.cmd == "evaluate":
+        import json
+        net = lg.grid_surrogate_load(a.weights.encode(), 1)
+        if not net:
+            raise SystemExit(f"cannot read {a.weights}")
+        modes = tuple(a.modes.split(","))
+        log(f"evaluating {a.count} held-out potentials (seed {HELDOUT_SEED}) on {a.n}^3: {modes}")
+        recs = evaluate(lg, net, a.count, a.n, modes, first=a.first)
+        summ = summarize(recs, modes)
+        log(json.dumps(summ, indent=1))
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(dict(summary=summ, records=recs), f, indent=1)
+        lg.grid_surrogate_free(net)
+    if a.cmd == "train":
+        log(f"training set: {a.count} potentials (seed {TRAIN_SEED}), {COARSE}^3, box {BOX} bohr")
+        V, T, st = make_dataset(lg, TRAIN_SEED, a.count)
+        log(f"  relaxation steps: median {int(np.median(st))}, max {st.max()}")
+        Vh, Th, sth = make_dataset(lg, HELDOUT_SEED, a.heldout)
+        net, hist, sec = train(lg, V, T, a.epochs, a.lr, a.batch)
+        log(f"trained {a.epochs} epochs in {sec:.1f} s; loss per epoch {np.array2string(hist, precision=4)}")
+        lh = lg.grid_surrogate_loss(net, len(Vh), _dp(Vh), _dp(Th))
+        pred = np.zeros_like(Th)
+        lg.grid_surrogate_predict(net, len(Vh), _dp(Vh), _dp(pred))
+        ov = [overlap(pred[i], Th[i]) for i in range(len(Vh))]
+        log(f"held-out loss {lh:.4f}; overlap <net|psi0> median {np.median(ov):.4f} min {np.min(ov):.4f}")
+        if lg.grid_surrogate_save(net, a.out.encode()) != 0:
+            raise SystemExit(f"cannot write {a.out}")
+        log(f"saved {a.out}")
+        lg.grid_surrogate_free(net)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

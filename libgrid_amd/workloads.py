@@ -135,3 +135,33 @@ def wells_2d(seed: int = 0, count: int = 256, step: float = 0.2, sizes=(64, 96, 
         out.append((psi, v.reshape(1, n, n), dict(n=n, step=step, wells=nw, depth=depth.tolist(), width=width.tolist(),
                                                  centre=centre.tolist())))
     return out
+
+
+def wells_3d_params(seed: int, count: int, box: float):
+    """Seeded parameters of 3-D attractive Gaussian wells: 4-10 wells, depth U[0.05, 0.5] Eh,
+    width U[0.5, 3] bohr, centres uniform within the middle 60% of the box (per axis)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nw = int(rng.integers(4, 11))
+        out.append(dict(depth=rng.uniform(0.05, 0.5, nw), width=rng.uniform(0.5, 3.0, nw),
+                        centre=rng.uniform(-0.3 * box, 0.3 * box, (nw, 3)), box=box))
+    return out
+
+
+def wells_3d(prm, n: int, step: float):
+    """V on an n^3 grid (coordinates (i - n/2) step, as cgrid_map): sum of -depth exp(-d^2 / (2
+    width^2)) with d the minimum-image distance in the periodic box of side n step, so V is
+    smooth across the boundary.  Grids of the same box that share every f-th point sample the
+    same function there."""
+    box = n * step
+    assert abs(box - prm["box"]) < 1e-9 * box
+    x = axes(n, step)
+    v = np.zeros((n, n, n))
+    for d, w, c in zip(prm["depth"], prm["width"], prm["centre"]):
+        f = []
+        for a in range(3):
+            dx = (x - c[a] + 0.5 * box) % box - 0.5 * box
+            f.append(np.exp(-dx * dx / (2 * w * w)))
+        v -= d * (f[0][:, None, None] * f[1][None, :, None] * f[2][None, None, :])
+    return v
