@@ -304,6 +304,12 @@ void grid_surrogate_predict(grid_surrogate *s, INT n, const REAL *pot, REAL *amp
 /* copy n samples to the device, then train for epochs (Adam, cosine-decayed lr) */
 void grid_surrogate_set_training_data(grid_surrogate *s, INT n, const REAL *pot, const REAL *target);
 REAL grid_surrogate_train(grid_surrogate *s, INT epochs, REAL lr, INT seed, REAL *loss_history);
+/* as grid_surrogate_train, with flags: GRID_SURROGATE_AUGMENT draws every sample under a
+ * random symmetry of the periodic cube (48 point operations and the translations);
+ * GRID_SURROGATE_PROGRESS prints one line per epoch to stderr */
+#define GRID_SURROGATE_AUGMENT 1
+#define GRID_SURROGATE_PROGRESS 2
+REAL grid_surrogate_train_ex(grid_surrogate *s, INT epochs, REAL lr, INT seed, int flags, REAL *loss_history);
 int grid_surrogate_save(grid_surrogate *s, const char *path);
 grid_surrogate *grid_surrogate_load(const char *path, INT batch);
 /* starts: the network's (s may be NULL: Gaussian of fallback_width) and the plain Gaussian

@@ -77,6 +77,8 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
 #define HGRID_PF_POST_RED   0x0080u  /* post: partial sum of |psi|^2 of the stored values */
 #define HGRID_PF_NOSTORE    0x0100u  /* reduce only; do not write the array */
 #define HGRID_PF_PRE_VEFF   0x0200u  /* pre: rho_out = vext + g |psi|^2 (GPE potential) instead of |psi|^2 */
+#define HGRID_PF_KIN_REAL   0x0400u  /* with MID_KIN: the tables are real (imaginary time); the factor is
+                                        applied as an FP64 real scale */
 
 /* Reduction channels (partials[chan * nblocks + block]). */
 #define HGRID_RED_NORM   0   /* sum |psi|^2 of the pass input   (PRE_RED) */
@@ -259,6 +261,9 @@ int hgrid_nn_wprep(const float *w, int cout, int cin, void *wb, void *wt);
 int hgrid_nn_adam(float *w, const float *g, float *m1, float *m2, long n, float lr, float b1, float b2, float eps, int t);
 /* input tensor [m][8]: vscale * v[m], then zeros */
 int hgrid_nn_input(const float *v, float vscale, void *x0, long nvox);
+/* one S^3 sample under a symmetry of the periodic cube (axis permutation perm < 6, reflections
+ * in the bits of flips, then a periodic shift) */
+int hgrid_nn_symgather(const float *src, float *dst, int lgS, int perm, int flips, int s0, int s1, int s2);
 /* periodic trilinear interpolation of an S^3 field onto n^3 (n = fct S) as a real wavefunction */
 int hgrid_nn_to_grid(const float *a, int lgS, int fct, long n, void *psi, int prec);
 

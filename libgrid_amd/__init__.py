@@ -277,6 +277,7 @@ PROTOTYPES = {
     "grid_surrogate_predict": (V, [C.c_void_p, L, C.POINTER(D), C.POINTER(D)]),
     "grid_surrogate_set_training_data": (V, [C.c_void_p, L, C.POINTER(D), C.POINTER(D)]),
     "grid_surrogate_train": (D, [C.c_void_p, L, D, L, C.POINTER(D)]),
+    "grid_surrogate_train_ex": (D, [C.c_void_p, L, D, L, C.c_int, C.POINTER(D)]),
     "grid_surrogate_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "grid_surrogate_load": (C.c_void_p, [C.c_char_p, L]),
     "grid_wf_warm_start": (V, [WF, RG, C.c_void_p, D]),
@@ -297,6 +298,7 @@ PROTOTYPES = {
     "hgrid_mem_info": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "hgrid_malloc": (C.c_int, [C.POINTER(VP), C.c_size_t]),
     "hgrid_free": (C.c_int, [VP]),
+    "hgrid_nn_symgather": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hgrid_memcpy_h2d": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_memcpy_d2h": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_memcpy_d2d": (C.c_int, [VP, VP, C.c_size_t]),

@@ -32,6 +32,11 @@ template <typename C> LG_CX C cneg(C a) { return mk<C>(-a.x, -a.y); }
 template <typename C> LG_CX C cconj(C a) { return mk<C>(a.x, -a.y); }
 template <typename C> LG_CX C cmul(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 template <typename C> LG_CX C cscale(C a, typename real_of<C>::type s) { return mk<C>(a.x * s, a.y * s); }
+// a * s with the product formed in FP64 and rounded once to a's precision
+template <typename C> LG_CX C cscale_d(C a, double s) {
+  using R = typename real_of<C>::type;
+  return mk<C>((R)((double)a.x * s), (R)((double)a.y * s));
+}
 // an FP64 complex rounded once to the precision of C
 template <typename C> LG_CX C round_to(double2 k) {
   using R = typename real_of<C>::type;

@@ -136,10 +136,11 @@ template <> __device__ __forceinline__ double2 phase_factor<double2>(double cr, 
   return make_double2(m * c, m * s);
 }
 
-// exp(x) for a real exponent: FP64 for double storage, FP32 after an FP64 argument otherwise
-template <typename C> __device__ __forceinline__ double real_decay(double x);
-template <> __device__ __forceinline__ double real_decay<double2>(double x) { return exp_f64(x); }
-template <> __device__ __forceinline__ double real_decay<float2>(double x) { return (double)expf((float)x); }
+// exp(x) for a real exponent, in FP64 for either storage precision: an FP32-rounded factor
+// would be the same per element every step, i.e. a fixed perturbation of V by about
+// 6e-8 / dt, which moves the FP32 fixed point of imaginary-time relaxation by ~1e-9 Eh.
+// Applied with cscale_d, only the product is rounded, and that rounding follows the data.
+__device__ __forceinline__ double real_decay(double x) { return exp_f64(x); }
 
 template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, double ci, double vr, double vi) {
   // FP64 argument, reduced mod 2 pi in FP64, then FP32 sincos / exp
@@ -250,11 +251,24 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     }
   }
   if constexpr (KIND == KIND_KIN) {
-    // FP64 tables whatever the storage precision; the factor is formed in FP64
+    // FP64 tables whatever the storage precision; the factor is formed in FP64.  Real
+    // factors (imaginary time) multiply in FP64 and only the product is rounded, so FP32
+    // storage sees no fixed per-element error in K; complex ones are rounded once to C.
     const double2 *ka = (const double2 *)a.ktab_a;
     const double2 f = cmul(((const double2 *)a.ktab_1)[c1], ((const double2 *)a.ktab_2)[c2]);
+    if (fl & HGRID_PF_KIN_REAL) {
+      // groups of four behind scheduling fences keep few FP64 factors live at once
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
+      for (int j0 = 0; j0 < R; j0 += 4) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = j0; j < (j0 + 4 < R ? j0 + 4 : R); ++j) v[j] = cscale_d(v[j], ka[t + TPP * j].x * f.x);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
+    }
   } else if constexpr (KIND == KIND_POT || KIND == KIND_GPE) {
     const Rl mid = (Rl)a.mid;
     const auto rpot = make_rsrc(a.pot, (unsigned long)ntot * CS);
@@ -282,7 +296,7 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
       if (a.c_im == 0.0) {
         // imaginary time: a real decay factor exp(c_re * V)
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = cscale(v[j], (Rl)((double)mid * real_decay<C>(a.c_re * vr[j])));
+        for (int j = 0; j < R; ++j) v[j] = cscale_d(v[j], (double)mid * real_decay(a.c_re * vr[j]));
       } else {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, vr[j], 0.0));

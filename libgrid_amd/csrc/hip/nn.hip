@@ -416,6 +416,30 @@ __global__ void k_input(const float *__restrict__ v, float vscale, bf16 *__restr
   *(bf16x8 *)(x0 + m * 8) = o;
 }
 
+// dst = src under one symmetry of the periodic cube: dst[c] = src[c'] with the index triple
+// c' = P(c) (axis permutation perm < 6), reflected i -> -i mod S on the axes in the bits of
+// flips, then shifted by (s0, s1, s2) mod S.  The Schroedinger problem on the periodic box is
+// invariant under these maps, so (V, psi_0) pairs stay pairs.
+__global__ void k_symgather(const float *__restrict__ src, float *__restrict__ dst, int lgS, int perm, int flips, int s0,
+                            int s1, int s2) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int S = 1 << lgS, msk = S - 1;
+  if (m >= (1L << (3 * lgS))) return;
+  const int c[3] = {(int)(m >> (2 * lgS)), (int)((m >> lgS) & msk), (int)(m & msk)};
+  static constexpr int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  int q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int v = c[P[perm][a]];
+    if ((flips >> a) & 1) v = (S - v) & msk;
+    q[a] = v;
+  }
+  q[0] = (q[0] + s0) & msk;
+  q[1] = (q[1] + s1) & msk;
+  q[2] = (q[2] + s2) & msk;
+  dst[m] = src[((long)q[0] << (2 * lgS)) + ((long)q[1] << lgS) + q[2]];
+}
+
 // periodic trilinear interpolation of a coarse S^3 field onto the n^3 grid (n = S * fct):
 // fine index i sits at coarse coordinate i / fct.  psi gets the real value, zero imaginary part.
 __global__ void k_to_grid(const float *__restrict__ a, int lgS, int fct, long n, void *__restrict__ psi, int prec) {
@@ -576,6 +600,13 @@ extern "C" int hgrid_nn_adam(float *w, const float *g, float *m1, float *m2, lon
 extern "C" int hgrid_nn_input(const float *v, float vscale, void *x0, long nvox) {
   hipLaunchKernelGGL(k_input, dim3(blocks(nvox, 256)), dim3(256), 0, lg::stream(), v, vscale, (bf16 *)x0, nvox);
   return lg::check_launch("nn_input");
+}
+
+extern "C" int hgrid_nn_symgather(const float *src, float *dst, int lgS, int perm, int flips, int s0, int s1, int s2) {
+  if (perm < 0 || perm > 5 || lgS < 1) return lg::fail("nn_symgather: arguments");
+  hipLaunchKernelGGL(k_symgather, dim3(blocks(1L << (3 * lgS), 256)), dim3(256), 0, lg::stream(), src, dst, lgS, perm,
+                     flips, s0, s1, s2);
+  return lg::check_launch("nn_symgather");
 }
 
 extern "C" int hgrid_nn_to_grid(const float *a, int lgS, int fct, long n, void *psi, int prec) {

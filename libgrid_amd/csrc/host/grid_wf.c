@@ -262,6 +262,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
     p.dir1 = -1;
     p.dir2 = +1;
     p.flags |= HGRID_PF_MID_KIN;
+    if (sa->tm_im == 0.0) p.flags |= HGRID_PF_KIN_REAL; /* exp(tm k^2) with real tm: real tables */
     set_kin(&p, kt, &s, g);
     if (half == 0 && sa->want_energy) {
       p.flags |= HGRID_PF_MID_KRED;

@@ -12,6 +12,13 @@
  * energies scatter by a few 1e-9 per step, which makes one-step ratios meaningless near
  * 1e-8).  A decrease D2 that is negative and smaller than tol means rounding dominates the
  * energy change: that also ends the relaxation.
+ *
+ * The estimate is trusted only when two successive ratios of one stride agree within a factor
+ * of two (the larger one is used), and windows are at least LG_RELAX_MIN_STRIDE steps long.
+ * Both guard against a fast transient ending on top of a slow tail (E_n - E_inf = a rho_1^n +
+ * b rho_2^n, rho_1 << rho_2): short windows see q ~ rho_1^m while b is still far above tol,
+ * and the slow part's decrease per window can be below tol.  An FP64 stage started from an
+ * FP32-relaxed state looks exactly like that.
  */
 #ifndef LG_RELAX_RULE_H
 #define LG_RELAX_RULE_H
@@ -20,11 +27,12 @@
 #define LG_RELAX_FN static inline
 #endif
 
-#define LG_RELAX_MIN_STRIDE 4
+#define LG_RELAX_MIN_STRIDE 16
 #define LG_RELAX_MAX_STRIDE 4096
 
 typedef struct {
   double c0, c1; /* energies at the two previous checkpoints */
+  double qprev;  /* ratio at the previous check of this stride (1 if none) */
   long m;        /* stride */
   long next;     /* step of the next checkpoint */
   int nchk;      /* checkpoints taken at this stride (0, 1, 2) */
@@ -32,6 +40,7 @@ typedef struct {
 
 LG_RELAX_FN void lg_relax_init(lg_relax_state *s) {
   s->c0 = s->c1 = 0.0;
+  s->qprev = 1.0;
   s->m = LG_RELAX_MIN_STRIDE;
   s->next = 0;
   s->nchk = 0;
@@ -49,15 +58,19 @@ LG_RELAX_FN int lg_relax_feed(lg_relax_state *s, long n, double e, double tol) {
     return 0;
   }
   const double d1 = s->c0 - s->c1, d2 = s->c1 - e;
-  if (d2 <= 0.0 && -d2 < tol) return 1;
   const int geometric = d1 > 0.0 && d2 > 0.0 && d2 < d1;
   const double q = geometric ? d2 / d1 : 1.0;
-  if (geometric && d2 < tol && d2 * q < tol * (1.0 - q)) return 1;
+  if (d2 <= 0.0 && -d2 < tol) return 1;
+  const int consistent = geometric && s->qprev < 1.0 && q < 2.0 * s->qprev && s->qprev < 2.0 * q;
+  const double qq = q > s->qprev ? q : s->qprev;
+  if (consistent && d2 < tol && d2 * qq < tol * (1.0 - qq)) return 1;
+  s->qprev = q;
   if (d1 > 0.0 && d2 > 0.0 && q > 0.5 && s->m < LG_RELAX_MAX_STRIDE) {
     /* double the stride: the checkpoints E(n - 2m) and E(n) are the new pair */
     s->c1 = e;
     s->m *= 2;
     s->next = n + s->m;
+    s->qprev = 1.0;
     return 0;
   }
   s->c0 = s->c1;

@@ -357,9 +357,17 @@ void grid_surrogate_set_training_data(grid_surrogate *s, INT nsamples, const REA
 }
 
 /* Epochs of Adam over the resident training set in shuffled minibatches (Fisher-Yates from a
- * seeded SplitMix64 stream); the learning rate rises linearly over the first 5% of the steps
+ * seeded SplitMix64 stream; with augment, each sample drawn under a random symmetry of the
+ * periodic cube: axis permutation, reflections, translation); the learning rate rises linearly over the first 5% of the steps
  * (at least 20) and decays by a cosine from lr to lr / 20 over the run.  loss_history (may be NULL) receives each epoch's mean training loss; returns the last. */
 REAL grid_surrogate_train(grid_surrogate *s, INT epochs, REAL lr, INT seed, REAL *loss_history) {
+  return grid_surrogate_train_ex(s, epochs, lr, seed, 0, loss_history);
+}
+
+REAL grid_surrogate_train_ex(grid_surrogate *s, INT epochs, REAL lr, INT seed, int flags, REAL *loss_history) {
+  const int augment = flags & GRID_SURROGATE_AUGMENT;
+  grid_timer tm;
+  grid_timer_start(&tm);
   if (!s->set_n) lg_fatal("grid_surrogate_train: no training data (grid_surrogate_set_training_data)");
   long *perm = (long *)malloc(sizeof(long) * s->set_n);
   unsigned long long st = 0xd1b54a32d192ed03ULL * (unsigned long long)(seed + 7);
@@ -382,8 +390,18 @@ REAL grid_surrogate_train(grid_surrogate *s, INT epochs, REAL lr, INT seed, REAL
       const long i0 = bi * s->batch;
       const long nb = (s->set_n - i0) < s->batch ? (s->set_n - i0) : s->batch;
       for (long k = 0; k < nb; ++k) {
-        lg_hip(hgrid_memcpy_d2d(s->vin + k * s->nv, s->set_v + perm[i0 + k] * s->nv, sizeof(float) * s->nv), "batch");
-        lg_hip(hgrid_memcpy_d2d(s->tgt + k * s->nv, s->set_t + perm[i0 + k] * s->nv, sizeof(float) * s->nv), "batch");
+        const float *sv = s->set_v + perm[i0 + k] * s->nv, *stg = s->set_t + perm[i0 + k] * s->nv;
+        if (augment) {
+          /* the same random symmetry of the periodic cube for V and its ground state */
+          const unsigned long long r = lg_splitmix64(&st);
+          const int S = 1 << s->lgS, ax = (int)(r % 6), fl = (int)((r >> 8) & 7);
+          const int s0 = (int)((r >> 16) % S), s1 = (int)((r >> 32) % S), s2 = (int)((r >> 48) % S);
+          lg_hip(hgrid_nn_symgather(sv, s->vin + k * s->nv, s->lgS, ax, fl, s0, s1, s2), "nn_symgather");
+          lg_hip(hgrid_nn_symgather(stg, s->tgt + k * s->nv, s->lgS, ax, fl, s0, s1, s2), "nn_symgather");
+        } else {
+          lg_hip(hgrid_memcpy_d2d(s->vin + k * s->nv, sv, sizeof(float) * s->nv), "batch");
+          lg_hip(hgrid_memcpy_d2d(s->tgt + k * s->nv, stg, sizeof(float) * s->nv), "batch");
+        }
       }
       forward(s, s->nv * nb);
       sum += backward(s, nb) * (double)nb;
@@ -393,6 +411,11 @@ REAL grid_surrogate_train(grid_surrogate *s, INT epochs, REAL lr, INT seed, REAL
     }
     last = sum / (double)s->set_n;
     if (loss_history) loss_history[ep] = last;
+    if (flags & GRID_SURROGATE_PROGRESS) {
+      fprintf(stderr, "libgrid: surrogate epoch %ld/%ld loss %.6f (%.1f s)\n", (long)ep + 1, (long)epochs, last,
+              grid_timer_wall_clock_time(&tm));
+      fflush(stderr);
+    }
   }
   free(perm);
   return last;

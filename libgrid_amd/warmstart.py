@@ -121,15 +121,31 @@ def make_dataset(lg, seed, count, n=COARSE, tol=1e-7, max_steps=20000, every=100
     return V, T, steps
 
 
-def train(lg, V, T, epochs=30, lr=1e-3, batch=8, seed=0):
+def train(lg, V, T, epochs=30, lr=1e-3, batch=8, seed=0, budget=None, augment=True):
+    """Adam over (V, T); with a time budget (seconds) the epoch count is cut to what fits,
+    from the measured time of a few minibatches."""
     net = lg.grid_surrogate_alloc(V.shape[1], batch, VSCALE, seed)
     n = V.shape[0]
+    if budget:
+        vb, tb = np.ascontiguousarray(V[:batch]), np.ascontiguousarray(T[:batch])
+        lg.grid_surrogate_gradient(net, batch, _dp(vb), _dp(tb), None)
+        lg.grid_hip_sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            lg.grid_surrogate_gradient(net, batch, _dp(vb), _dp(tb), None)
+        lg.grid_hip_sync()
+        per_batch = (time.perf_counter() - t0) / 3
+        per_epoch = per_batch * ((n + batch - 1) // batch)
+        fit = max(1, int(budget / per_epoch))
+        log(f"  {per_batch * 1e3:.1f} ms per minibatch of {batch} (forward + backward), {per_epoch:.1f} s per epoch; "
+            f"epochs {epochs} -> {min(epochs, fit)}")
+        epochs = min(epochs, fit)
     lg.grid_surrogate_set_training_data(net, n, _dp(np.ascontiguousarray(V)), _dp(np.ascontiguousarray(T)))
     hist = np.zeros(epochs)
     t0 = time.perf_counter()
-    lg.grid_surrogate_train(net, epochs, lr, seed, _dp(hist))
+    lg.grid_surrogate_train_ex(net, epochs, lr, seed, (1 if augment else 0) | 2, _dp(hist))
     lg.grid_hip_sync()
-    return net, hist, time.perf_counter() - t0
+    return net, hist[:epochs], time.perf_counter() - t0
 
 
 def overlap(a, b):
@@ -197,6 +213,9 @@ def main(argv=None):
     t.add_argument("--epochs", type=int, default=30)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--budget", type=float, default=None, help="training time budget in seconds")
+    t.add_argument("--tol", type=float, default=1e-6, help="energy tolerance of the training ground states")
+    t.add_argument("--max-steps", type=int, default=8000)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
     e.add_argument("--count", type=int, default=200)
@@ -222,11 +241,11 @@ def main(argv=None):
         lg.grid_surrogate_free(net)
     if a.cmd == "train":
         log(f"training set: {a.count} potentials (seed {TRAIN_SEED}), {COARSE}^3, box {BOX} bohr")
-        V, T, st = make_dataset(lg, TRAIN_SEED, a.count)
-        log(f"  relaxation steps: median {int(np.median(st))}, max {st.max()}")
-        Vh, Th, sth = make_dataset(lg, HELDOUT_SEED, a.heldout)
-        net, hist, sec = train(lg, V, T, a.epochs, a.lr, a.batch)
-        log(f"trained {a.epochs} epochs in {sec:.1f} s; loss per epoch {np.array2string(hist, precision=4)}")
+        V, T, st = make_dataset(lg, TRAIN_SEED, a.count, tol=a.tol, max_steps=a.max_steps)
+        log(f"  relaxation steps: median {int(np.median(st))}, max {st.max()}, capped {int(np.sum(st >= a.max_steps))}")
+        Vh, Th, sth = make_dataset(lg, HELDOUT_SEED, a.heldout, tol=a.tol, max_steps=a.max_steps)
+        net, hist, sec = train(lg, V, T, a.epochs, a.lr, a.batch, budget=a.budget)
+        log(f"trained {len(hist)} epochs in {sec:.1f} s; loss per epoch {np.array2string(hist, precision=4)}")
         lh = lg.grid_surrogate_loss(net, len(Vh), _dp(Vh), _dp(Th))
         pred = np.zeros_like(Th)
         lg.grid_surrogate_predict(net, len(Vh), _dp(Vh), _dp(pred))

@@ -110,3 +110,33 @@ def test_training_reduces_loss(lg):
         assert l1 < 0.2 * l0
     finally:
         lg.grid_surrogate_free(s)
+
+
+def test_symgather_matches_numpy(lg):
+    """The training augmentation: one S^3 sample under an axis permutation, reflections
+    i -> -i mod S and a periodic shift, against the same map built with numpy indices."""
+    case("symgather 8^3")
+    S, lgS = 8, 3
+    rng = np.random.default_rng(4)
+    src = rng.normal(size=(S, S, S)).astype(np.float32)
+    nbytes = src.nbytes
+    d_src, d_dst = C.c_void_p(), C.c_void_p()
+    assert lg.hgrid_malloc(C.byref(d_src), nbytes) == 0 and lg.hgrid_malloc(C.byref(d_dst), nbytes) == 0
+    try:
+        assert lg.hgrid_memcpy_h2d(d_src, src.ctypes.data, nbytes) == 0
+        P = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+        c = np.indices((S, S, S))
+        for perm, flips, sh in [(0, 0, (0, 0, 0)), (3, 5, (1, 7, 2)), (5, 2, (4, 0, 3)), (1, 7, (6, 6, 6))]:
+            assert lg.hgrid_nn_symgather(d_src, d_dst, lgS, perm, flips, *sh) == 0
+            out = np.empty_like(src)
+            assert lg.hgrid_memcpy_d2h(out.ctypes.data, d_dst, nbytes) == 0
+            q = []
+            for a in range(3):
+                v = c[P[perm][a]]
+                if (flips >> a) & 1:
+                    v = (S - v) % S
+                q.append((v + sh[a]) % S)
+            np.testing.assert_array_equal(out, src[q[0], q[1], q[2]])
+    finally:
+        lg.hgrid_free(d_src)
+        lg.hgrid_free(d_dst)

@@ -87,3 +87,17 @@ def test_rise_below_tol_ends(rule):
     e = np.full(100, -0.5)
     e[40:] += 1e-12
     assert 0 < rule(e, 1e-10) < 100
+
+
+def test_fast_transient_on_slow_tail(rule):
+    """E_n - E_inf = a 0.3^n + b 0.999^n with b = 3e-9 >> tol: the fast part hides the slow tail
+    from a single check (q ~ 0.3^m); the rule must still relax the slow part below a few tol."""
+    tol = 1e-10
+    k = np.arange(300000)
+    for a in (1e-6, 1e-8, 3e-9):
+        e = -0.2 + a * 0.3 ** k + 3e-9 * 0.999 ** k
+        n = rule(e, tol)
+        assert n > 0
+        rem = e[n] + 0.2
+        print(f"a {a:.0e}: stopped at {n}, remaining {rem:.2e}")
+        assert rem < 3 * tol
