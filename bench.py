@@ -193,6 +193,79 @@ def warm_start_bench(lib, count):
                 grid=[WS.FINE] * 3, box_bohr=WS.BOX, dt=WS.DT, tol_Eh=1e-10)
 
 
+def fp32_512_bench(lib, LG, steps=20, warmup=3):
+    """The 512^3 complex64 real-time GPE step of the vortex-ring workload (dt 0.02, g 0.1,
+    harmonic trap omega 0.1, spacing 0.15): FP64 twiddles, arguments and reductions.  The step's
+    cost does not depend on the state, so a Gaussian condensate built on the host stands in for
+    the ring (its accuracy run is tests/test_gpu_configs.py)."""
+    n, step, g, dt, omega = 512, 0.15, 0.1, 0.02, 0.1
+    w = lib.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"ring")
+    vr = lib.rgrid_alloc(n, n, n, step, LG.boundary("rgrid_value_outside_periodic"), None, b"trap")
+    x = (np.arange(n) - n // 2) * step
+    r2 = (x * x)[:, None, None] + (x * x)[None, :, None] + (x * x)[None, None, :]
+    LG.set_rgrid(vr, 0.5 * omega * omega * r2)
+    del r2
+    lib.cgrid_set_precision(w.contents.grid, b"\x01")
+    lib.grid_wf_gaussian_start(w, 10.0)
+    for _ in range(warmup):
+        lib.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(dt), b"\x00")
+    lib.grid_hip_sync()
+    t = LG.HipTimer()
+    t.start()
+    for _ in range(steps):
+        lib.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(dt), b"\x00")
+    ms = t.stop() / steps
+    t.close()
+    lib.grid_wf_free(w)
+    lib.rgrid_free(vr)
+    N = n ** 3
+    model = 200 * N  # (4 FFTs x 3 sweeps x 2 x 8 B + 8 B potential) per point
+    out = dict(workload="vortex_ring 512^3 complex64 real-time GPE step (dt 0.02, g 0.1)", ms_per_step=ms,
+               steps_per_s=1000.0 / ms, model_bytes_per_step=model,
+               model_frac=model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS)
+    log(f"fp32 512^3: {ms:.3f} ms/step, {1000 / ms:.1f} steps/s, model-roofline frac {out['model_frac']:.3f}")
+    return out
+
+
+def batch2d_bench(lib, LG, count=256, tol=1e-8, dt=0.1):
+    """The batched 2-D workload: 256 seeded problems (n in 64..256, 3-8 Gaussian wells, spacing
+    0.2) relaxed to tol in one persistent launch (autotuned tile)."""
+    from libgrid_amd import workloads as W
+
+    problems = W.wells_2d(seed=0, count=count)
+    wfs, pots = [], []
+    for psi, v, prm in problems:
+        nn = prm["n"]
+        w = lib.grid_wf_alloc(1, nn, nn, prm["step"], 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"b2d")
+        LG.set_cgrid(w.contents.grid, psi)
+        r = lib.rgrid_alloc(1, nn, nn, prm["step"], LG.boundary("rgrid_value_outside_periodic"), None, b"v")
+        LG.set_rgrid(r, v)
+        wfs.append(w)
+        pots.append(r)
+    warr, parr = (LG.WF * count)(*wfs), (LG.RG * count)(*pots)
+    report = lib.grid_batch2d_autotune(count, warr, parr, dt, 20).decode()
+    e = (C.c_double * count)()
+    st = (C.c_long * count)()
+    t0 = time.perf_counter()
+    secs = lib.grid_wf_batch2d_relax(count, warr, parr, dt, tol, 200000, e, st)
+    wall = time.perf_counter() - t0
+    ts, te = (C.c_double * count)(), (C.c_double * count)()
+    lib.grid_batch2d_solve_times(ts, te)
+    busy = float(sum(te[i] - ts[i] for i in range(count)))
+    cus = lib.hgrid_b2d_cus()
+    steps = np.array(st[:])
+    for w in wfs:
+        lib.grid_wf_free(w)
+    for r in pots:
+        lib.rgrid_free(r)
+    out = dict(workload=f"{count} x 2-D wells (n in 64..256, spacing 0.2, dt {dt}, tol {tol})", solves_per_s=count / secs,
+               launch_s=secs, wall_s_incl_packing=wall, steps_median=int(np.median(steps)), steps_max=int(steps.max()),
+               utilisation=busy / (cus * secs) if cus > 0 else None, autotune=report)
+    log(f"batch2d: {count / secs:.1f} solves/s ({secs:.3f} s launch), steps median {out['steps_median']} max "
+        f"{out['steps_max']}, utilisation {out['utilisation']:.3f}")
+    return out
+
+
 def cpu_baseline(psi, vext, prm, steps):
     from oracle import oracle as orc
 
@@ -276,6 +349,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gs", action="store_true", help="skip the time-to-ground-state runs")
     ap.add_argument("--gs-max-steps", type=int, default=20000)
+    ap.add_argument("--no-rows", action="store_true", help="skip the FP32 512^3 and batched 2-D measurements")
     ap.add_argument("--ws-count", type=int, default=2,
                     help="held-out 256^3 wells potentials for the warm-start time-to-ground-state comparison")
     args = ap.parse_args()
@@ -339,6 +413,11 @@ def main():
     if not args.no_gs and args.ws_count > 0:
         ws = warm_start_bench(lib, args.ws_count)
 
+    rows = {}
+    if not args.no_rows:
+        rows["fp32_512"] = fp32_512_bench(lib, LG)
+        rows["batch2d"] = batch2d_bench(lib, LG)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -374,6 +453,7 @@ def main():
         "energy": energy,
         "time_to_ground_state": gs,
         "warm_start": ws,
+        "other_rows": rows,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -299,6 +299,7 @@ PROTOTYPES = {
     "hgrid_malloc": (C.c_int, [C.POINTER(VP), C.c_size_t]),
     "hgrid_free": (C.c_int, [VP]),
     "hgrid_nn_symgather": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "hgrid_b2d_cus": (C.c_int, []),
     "hgrid_memcpy_h2d": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_memcpy_d2h": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_memcpy_d2d": (C.c_int, [VP, VP, C.c_size_t]),

@@ -257,14 +257,19 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     const double2 *ka = (const double2 *)a.ktab_a;
     const double2 f = cmul(((const double2 *)a.ktab_1)[c1], ((const double2 *)a.ktab_2)[c2]);
     if (fl & HGRID_PF_KIN_REAL) {
-      // groups of four behind scheduling fences keep few FP64 factors live at once
+      if constexpr (std::is_same<C, float2>::value) {
+        // FP32 data: groups of four behind scheduling fences keep few FP64 factors live
 #pragma unroll
-      for (int j0 = 0; j0 < R; j0 += 4) {
+        for (int j0 = 0; j0 < R; j0 += 4) {
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int j = j0; j < (j0 + 4 < R ? j0 + 4 : R); ++j) v[j] = cscale_d(v[j], ka[t + TPP * j].x * f.x);
+        }
         __builtin_amdgcn_sched_barrier(0);
+      } else {
 #pragma unroll
-        for (int j = j0; j < (j0 + 4 < R ? j0 + 4 : R); ++j) v[j] = cscale_d(v[j], ka[t + TPP * j].x * f.x);
+        for (int j = 0; j < R; ++j) v[j] = cscale_d(v[j], ka[t + TPP * j].x * f.x);
       }
-      __builtin_amdgcn_sched_barrier(0);
     } else {
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
