@@ -72,6 +72,8 @@
 /* Device storage precision of a complex grid (this build's addition) */
 #define GRID_PREC_DOUBLE 0 /* complex128 on the device (default) */
 #define GRID_PREC_SINGLE 1 /* complex64 on the device; FP64 twiddles, phase arguments, reductions */
+#define GRID_PREC_SINGLE_STORAGE 2 /* complex64 on the device; the FFT/propagation passes do all their
+                                      arithmetic in FP64 and round only on store */
 
 struct grid_dev;
 
@@ -188,7 +190,8 @@ void cgrid_inverse_fft_norm2(cgrid *grid);                                      
 REAL complex cgrid_value_at_index(cgrid *grid, INT i, INT j, INT k);            /* :1960 */
 void cgrid_value_to_index(cgrid *grid, INT i, INT j, INT k, REAL complex value);
 REAL cgrid_fft_laplace_expectation_value(cgrid *grid);                          /* src/cgrid_diff.c:817 */
-/* device storage precision (GRID_PREC_DOUBLE / GRID_PREC_SINGLE); this build's addition */
+/* device storage precision (GRID_PREC_DOUBLE / GRID_PREC_SINGLE / GRID_PREC_SINGLE_STORAGE); this
+ * build's addition */
 void cgrid_set_precision(cgrid *grid, char prec);
 char cgrid_get_precision(cgrid *grid);
 /* host/device coherence for code that touches grid->value directly (this build's addition) */

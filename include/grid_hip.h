@@ -63,6 +63,8 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
 /* Precision of a device complex array. */
 #define HGRID_C128 0   /* double2 */
 #define HGRID_C64  1   /* float2 (twiddles and phase tables are computed in FP64, stored FP32) */
+#define HGRID_C64X 2   /* float2 storage; the pass kernels widen to double2 and do all arithmetic in FP64
+                          (FP64 twiddles); other kernels treat it as float2 */
 
 /* Pass flags (bit set).  Stage order inside one pass:
  *   load [* *scale] -> PRE ops -> FFT #1 -> MID ops -> [FFT #2] -> * post -> POST ops -> store */
@@ -91,7 +93,7 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
 typedef struct hgrid_pass {
   void *in;              /* device array (complex, precision prec) */
   void *out;             /* may equal in (in-place) */
-  int prec;              /* HGRID_C128 / HGRID_C64 */
+  int prec;              /* HGRID_C128 / HGRID_C64 / HGRID_C64X */
   long n0, n1, n2;       /* squeezed shape; n2 is contiguous */
   int axis;              /* 0, 1 or 2 */
   int nfft;              /* 1 or 2 transforms along the axis in this pass */
@@ -230,9 +232,12 @@ int hgrid_b2d_launch(const hgrid_b2d_job *jobs, int njobs, int *queue, int varia
 int hgrid_nn_conv3(const void *x, int xs, int xo, int cin, const void *w, const float *bias, void *y, int ys, int yo,
                    int cout, const void *mask, int mks, int mko, int lgS, long nvox, int relu);
 /* partial[p][co][t][ci] of dW = sum_m dy[m][dyo + co] x[m + d_t][xo + ci]; sum the
- * hgrid_nn_wgrad_partials(nvox) partials with hgrid_nn_reduce. */
+ * hgrid_nn_wgrad_partials(nvox) partials with hgrid_nn_reduce.  bpartial (may be NULL)
+ * receives [chunk][co] column sums of dy (the bias gradient), hgrid_nn_wgrad_chunks(nvox)
+ * rows. */
 int hgrid_nn_conv3_wgrad(const void *dy, int dys, int dyo, int cout, const void *x, int xs, int xo, int cin, int lgS,
-                         long nvox, float *partial);
+                         long nvox, float *partial, float *bpartial);
+long hgrid_nn_wgrad_chunks(long nvox);
 long hgrid_nn_wgrad_partials(long nvox);
 long hgrid_nn_wgrad_chunk(long nvox);
 int hgrid_nn_reduce(const float *partial, long np, long n, float *out, int accumulate);

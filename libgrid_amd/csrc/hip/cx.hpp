@@ -37,8 +37,13 @@ template <typename C> LG_CX C cscale_d(C a, double s) {
   using R = typename real_of<C>::type;
   return mk<C>((R)((double)a.x * s), (R)((double)a.y * s));
 }
-// an FP64 complex rounded once to the precision of C
-template <typename C> LG_CX C round_to(double2 k) {
+// a complex rounded once to the precision of C (identity when the types agree)
+template <typename C, typename A> LG_CX C round_to(A k) {
+  using R = typename real_of<C>::type;
+  return mk<C>((R)k.x, (R)k.y);
+}
+// a stored value widened to the arithmetic type C (exact)
+template <typename C, typename A> LG_CX C widen(A k) {
   using R = typename real_of<C>::type;
   return mk<C>((R)k.x, (R)k.y);
 }

@@ -13,7 +13,8 @@ static long axis_len(long n0, long n1, long n2, int axis) { return axis == 0 ? n
 
 extern "C" long hgrid_pass_blocks(int prec, long n0, long n1, long n2, int axis) {
   const long L = axis_len(n0, n1, n2, axis);
-  const long P = prec == HGRID_C128 ? lg::pencils_per_block<double2>(L) : lg::pencils_per_block<float2>(L);
+  // the workgroup geometry follows the arithmetic type (double2 for FP32 storage with FP64 arithmetic)
+  const long P = prec == HGRID_C64 ? lg::pencils_per_block<float2>(L) : lg::pencils_per_block<double2>(L);
   if (P <= 0) return 0;
   const long npencils = (n0 * n1 * n2) / L;
   return (npencils + P - 1) / P;
@@ -36,5 +37,6 @@ extern "C" int hgrid_fft_pass(const hgrid_pass *p) {
   if (kind != lg::KIND_FFT && p->nfft != 2) return -1;
   if (p->prec == HGRID_C128) return lg::launch_f64(p, L, kind);
   if (p->prec == HGRID_C64) return lg::launch_f32(p, L, kind);
+  if (p->prec == HGRID_C64X) return lg::launch_f32s_f64(p, L, kind);
   return -1;
 }

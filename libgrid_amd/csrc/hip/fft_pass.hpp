@@ -153,12 +153,14 @@ template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, do
   return make_float2(m * c, m * s);
 }
 
-template <typename C, int L, bool CONTIG, int KIND>
+// C is the arithmetic type, S the storage type of psi (and of a complex potential): S = C, or
+// S = float2 with C = double2 ("FP32 storage, FP64 arithmetic": only the stores round).
+template <typename C, int L, bool CONTIG, int KIND, typename S = C>
 __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_pass a) {
   using Rl = typename real_of<C>::type;
   constexpr int TPP = Split<L>::TPP, R = Split<L>::R, NT = Geom<C, L>::NT, P = Geom<C, L>::P;
   constexpr int LDS_N = CONTIG ? Geom<C, L>::LDS_CONTIG : Geom<C, L>::LDS_STRIDED;
-  constexpr unsigned CS = sizeof(C);
+  constexpr unsigned CS = sizeof(S);  // bytes per stored element
   __shared__ Rl lds[LDS_N];
   __shared__ double red_lds[NT / 64][HGRID_RED_NCHAN];
 
@@ -210,7 +212,7 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
 
   C v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = active ? bload<C>(rin, e0 * CS, (unsigned)j * estep * CS) : mk<C>(0, 0);
+  for (int j = 0; j < R; ++j) v[j] = active ? widen<C>(bload<S>(rin, e0 * CS, (unsigned)j * estep * CS)) : mk<C>(0, 0);
 
   double acc[HGRID_RED_NCHAN] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
@@ -282,7 +284,7 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     if constexpr (KIND == KIND_POT) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        const C pv = bload<C>(rpot, (e0 + (unsigned)j * estep) * CS, 0);
+        const S pv = bload<S>(rpot, (e0 + (unsigned)j * estep) * CS, 0);
         v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, a.cons + (double)pv.x, (double)pv.y));
       }
     } else {
@@ -325,7 +327,7 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
   if (!(fl & HGRID_PF_NOSTORE) && active) {
     const auto rout = make_rsrc(a.out, (unsigned long)ntot * CS);
 #pragma unroll
-    for (int j = 0; j < R; ++j) bstore<C>(v[j], rout, e0 * CS, (unsigned)j * estep * CS);
+    for (int j = 0; j < R; ++j) bstore<S>(round_to<S>(v[j]), rout, e0 * CS, (unsigned)j * estep * CS);
   }
 
   if (fl & (HGRID_PF_PRE_RED | HGRID_PF_MID_KRED | HGRID_PF_POST_RED)) {
@@ -352,18 +354,18 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
   }
 }
 
-template <typename C, int L, bool CONTIG, int KIND> int launch(const hgrid_pass *p) {
+template <typename C, int L, bool CONTIG, int KIND, typename S> int launch(const hgrid_pass *p) {
   constexpr int NT = Geom<C, L>::NT, P = Geom<C, L>::P;
   const long npencils = (p->n0 * p->n1 * p->n2) / L;
   const long nblocks = (npencils + P - 1) / P;
   if (nblocks <= 0) return 0;
-  hipLaunchKernelGGL((fft_pass_kernel<C, L, CONTIG, KIND>), dim3((unsigned)nblocks), dim3(NT), 0, stream(), *p);
+  hipLaunchKernelGGL((fft_pass_kernel<C, L, CONTIG, KIND, S>), dim3((unsigned)nblocks), dim3(NT), 0, stream(), *p);
   return check_launch("fft_pass");
 }
 
-template <typename C, bool CONTIG, int KIND> int launch_len(const hgrid_pass *p, long L) {
+template <typename C, bool CONTIG, int KIND, typename S> int launch_len(const hgrid_pass *p, long L) {
 #define LG_CASE(n) \
-  case n: return launch<C, n, CONTIG, KIND>(p);
+  case n: return launch<C, n, CONTIG, KIND, S>(p);
   switch (L) {
     LG_FOR_EACH_LENGTH(LG_CASE)
     default: return -1;
@@ -371,13 +373,13 @@ template <typename C, bool CONTIG, int KIND> int launch_len(const hgrid_pass *p,
 #undef LG_CASE
 }
 
-template <typename C> int launch_prec(const hgrid_pass *p, long L, int kind) {
+template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long L, int kind) {
   const bool contig = p->axis == 2;
   switch (kind) {
-    case KIND_FFT: return contig ? launch_len<C, true, KIND_FFT>(p, L) : launch_len<C, false, KIND_FFT>(p, L);
-    case KIND_KIN: return contig ? launch_len<C, true, KIND_KIN>(p, L) : launch_len<C, false, KIND_KIN>(p, L);
-    case KIND_POT: return contig ? launch_len<C, true, KIND_POT>(p, L) : -1;
-    case KIND_GPE: return contig ? launch_len<C, true, KIND_GPE>(p, L) : -1;
+    case KIND_FFT: return contig ? launch_len<C, true, KIND_FFT, S>(p, L) : launch_len<C, false, KIND_FFT, S>(p, L);
+    case KIND_KIN: return contig ? launch_len<C, true, KIND_KIN, S>(p, L) : launch_len<C, false, KIND_KIN, S>(p, L);
+    case KIND_POT: return contig ? launch_len<C, true, KIND_POT, S>(p, L) : -1;
+    case KIND_GPE: return contig ? launch_len<C, true, KIND_GPE, S>(p, L) : -1;
     default: return -1;
   }
 }
@@ -395,5 +397,6 @@ template <typename C> long pencils_per_block(long L) {
 // per-precision entry points (one translation unit each, compiled in parallel)
 int launch_f64(const hgrid_pass *p, long L, int kind);
 int launch_f32(const hgrid_pass *p, long L, int kind);
+int launch_f32s_f64(const hgrid_pass *p, long L, int kind);  // float2 storage, FP64 arithmetic
 
 }  // namespace lg

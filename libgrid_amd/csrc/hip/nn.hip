@@ -133,7 +133,8 @@ constexpr int WG_MAXT = 12;     // ci tiles (Cin <= 192)
 template <int CT>
 __global__ void __launch_bounds__(256) k_conv3_wgrad(const bf16 *__restrict__ dy, int dys, int dyo,
                                                      const bf16 *__restrict__ x, int xs, int xo, int cin, int lgS,
-                                                     long nvox, long vc, int cout, float *__restrict__ partial) {
+                                                     long nvox, long vc, int cout, float *__restrict__ partial,
+                                                     float *__restrict__ bpartial) {
   __shared__ __align__(16) bf16 ys_[16 * CT * WG_P];
   __shared__ __align__(16) bf16 xs_[16 * WG_MAXT * WG_P];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -149,6 +150,9 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const bf16 *__restrict__ dy
   for (int c = 0; c < CT; ++c)
 #pragma unroll
     for (int j = 0; j < WG_MAXT; ++j) acc[c][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // the centre tap's workgroups also sum dY over their voxels: the bias gradient
+  const bool bias_wg = bpartial && tap == 13;
+  float bsum = 0.f;
 #pragma unroll 1
   for (long sb = v0; sb < v0 + vc; sb += WG_V) {
     __syncthreads();  // previous step's reads are done
@@ -177,6 +181,12 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const bf16 *__restrict__ dy
       for (int i = 0; i < 8; ++i) xs_[(c8 + i) * WG_P + vl] = t[i];
     }
     __syncthreads();
+    if (bias_wg && threadIdx.x < 16 * CT) {
+      const bf16 *row = ys_ + threadIdx.x * WG_P;
+      float s8 = 0.f;
+      for (int i = 0; i < WG_V; ++i) s8 += f(row[i]);
+      bsum += s8;
+    }
     const int kv = 32 * wave + 8 * g;  // this lane's 8 voxels of the wave's 32
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -189,6 +199,7 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const bf16 *__restrict__ dy
         }
     }
   }
+  if (bias_wg && threadIdx.x < 16 * CT) bpartial[(long)blockIdx.x * cout + co0 + threadIdx.x] = bsum;
   // partial[(chunk * 4 + wave)][co][tap][ci]
   float *pp = partial + ((long)blockIdx.x * 4 + wave) * ((long)cout * 27 * cin);
 #pragma unroll
@@ -211,18 +222,32 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const bf16 *__restrict__ dy
 __global__ void k_reduce(const float *__restrict__ partial, long np, long n, float *__restrict__ out, int accumulate) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  float s = 0.f;
-  for (long p = 0; p < np; ++p) s += partial[p * n + e];
-  out[e] = accumulate ? out[e] + s : s;
+  // eight independent running sums (partials p = 8k + i go to sum i), added in a fixed order
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  long p = 0;
+  for (; p + 8 <= np; p += 8)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s[i] += partial[(p + i) * n + e];
+  for (int i = 0; p < np; ++p, ++i) s[i] += partial[p * n + e];
+  const float t = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+  out[e] = accumulate ? out[e] + t : t;
 }
 
 // out[c] = sum_p partial[p * stride + c], c < ncols (fixed order)
-__global__ void k_reduce_strided(const float *__restrict__ partial, long np, int stride, int ncols, float *__restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= ncols) return;
+// one workgroup per column: thread t takes rows t, t + 256, ...; then a fixed tree
+__global__ void __launch_bounds__(256) k_reduce_strided(const float *__restrict__ partial, long np, int stride,
+                                                        float *__restrict__ out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
   float s = 0.f;
-  for (long p = 0; p < np; ++p) s += partial[p * stride + c];
-  out[c] = s;
+  for (long p = threadIdx.x; p < np; p += 256) s += partial[p * stride + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
 }
 
 // column sums of a [nvox][C] slice: partial[chunk][c]
@@ -502,8 +527,13 @@ extern "C" long hgrid_nn_wgrad_partials(long nvox) {
   return 4 * ((nvox + vc - 1) / vc);
 }
 
+extern "C" long hgrid_nn_wgrad_chunks(long nvox) {
+  const long vc = hgrid_nn_wgrad_chunk(nvox);
+  return (nvox + vc - 1) / vc;
+}
+
 extern "C" int hgrid_nn_conv3_wgrad(const void *dy, int dys, int dyo, int cout, const void *x, int xs, int xo, int cin,
-                                    int lgS, long nvox, float *partial) {
+                                    int lgS, long nvox, float *partial, float *bpartial) {
   if (cin % 8 || cin > 16 * WG_MAXT || cout % 16 || xo + cin > xs || dyo + cout > dys || nvox <= 0)
     return lg::fail("nn_conv3_wgrad: shape");
   const long vc = hgrid_nn_wgrad_chunk(nvox);
@@ -513,8 +543,8 @@ extern "C" int hgrid_nn_conv3_wgrad(const void *dy, int dys, int dyo, int cout, 
   auto *D = (const bf16 *)dy;
   auto *X = (const bf16 *)x;
   switch (ct) {
-    case 2: hipLaunchKernelGGL((k_conv3_wgrad<2>), grid, dim3(256), 0, lg::stream(), D, dys, dyo, X, xs, xo, cin, lgS, nvox, vc, cout, partial); break;
-    default: hipLaunchKernelGGL((k_conv3_wgrad<1>), grid, dim3(256), 0, lg::stream(), D, dys, dyo, X, xs, xo, cin, lgS, nvox, vc, cout, partial); break;
+    case 2: hipLaunchKernelGGL((k_conv3_wgrad<2>), grid, dim3(256), 0, lg::stream(), D, dys, dyo, X, xs, xo, cin, lgS, nvox, vc, cout, partial, bpartial); break;
+    default: hipLaunchKernelGGL((k_conv3_wgrad<1>), grid, dim3(256), 0, lg::stream(), D, dys, dyo, X, xs, xo, cin, lgS, nvox, vc, cout, partial, bpartial); break;
   }
   return lg::check_launch("nn_conv3_wgrad");
 }
@@ -526,8 +556,8 @@ extern "C" int hgrid_nn_reduce(const float *partial, long np, long n, float *out
 }
 
 extern "C" int hgrid_nn_reduce_strided(const float *partial, long np, int stride, int ncols, float *out) {
-  if (ncols <= 0 || ncols > 256 || stride < ncols || np <= 0) return lg::fail("nn_reduce_strided: shape");
-  hipLaunchKernelGGL(k_reduce_strided, dim3(1), dim3(256), 0, lg::stream(), partial, np, stride, ncols, out);
+  if (ncols <= 0 || stride < ncols || np <= 0) return lg::fail("nn_reduce_strided: shape");
+  hipLaunchKernelGGL(k_reduce_strided, dim3(ncols), dim3(256), 0, lg::stream(), partial, np, stride, out);
   return lg::check_launch("nn_reduce_strided");
 }
 

@@ -222,15 +222,21 @@ void cgrid_host_modified(cgrid *grid) {
 }
 
 void cgrid_set_precision(cgrid *grid, char prec) {
-  int p = prec == GRID_PREC_SINGLE ? HGRID_C64 : HGRID_C128;
+  int p = prec == GRID_PREC_SINGLE ? HGRID_C64 : prec == GRID_PREC_SINGLE_STORAGE ? HGRID_C64X : HGRID_C128;
   grid_dev *d = grid->dev;
   if (d->prec == p) return;
+  if (lg_prec_size(p) == lg_prec_size(d->prec)) {
+    /* FP32 storage either way: only the arithmetic of the passes changes */
+    d->prec = p;
+    free_ktab(&d->kin);
+    return;
+  }
   if (d->d) {
     if (d->valid & GV_DEV) {
       lg_c_flush(grid);
       void *nd;
       lg_hip(hgrid_malloc(&nd, d->n * lg_prec_size(p)), "hipMalloc (grid)");
-      if (p == HGRID_C64) lg_hip(hgrid_c128_to_c64(nd, d->d, d->n), "convert");
+      if (p != HGRID_C128) lg_hip(hgrid_c128_to_c64(nd, d->d, d->n), "convert");
       else lg_hip(hgrid_c64_to_c128(nd, d->d, d->n), "convert");
       lg_hip(hgrid_free(d->d), "hipFree");
       d->d = nd;
@@ -247,7 +253,10 @@ void cgrid_set_precision(cgrid *grid, char prec) {
   d->prec = p;
 }
 
-char cgrid_get_precision(cgrid *grid) { return grid->dev->prec == HGRID_C64 ? GRID_PREC_SINGLE : GRID_PREC_DOUBLE; }
+char cgrid_get_precision(cgrid *grid) {
+  const int p = grid->dev->prec;
+  return p == HGRID_C64 ? GRID_PREC_SINGLE : p == HGRID_C64X ? GRID_PREC_SINGLE_STORAGE : GRID_PREC_DOUBLE;
+}
 
 double lg_step_factor(INT nx, INT ny, INT nz, REAL step) {
   double f = 1.0;
@@ -359,10 +368,10 @@ void cgrid_copy(cgrid *dst, cgrid *src) {
   if (dst == src) return;
   void *s = lg_c_in_flushed(src);
   grid_dev *dd = dst->dev;
-  if (dd->prec != src->dev->prec) {
-    /* mixed precision copy */
+  if (lg_prec_size(dd->prec) != lg_prec_size(src->dev->prec)) {
+    /* copy between FP64 and FP32 storage */
     void *o = lg_c_out(dst);
-    if (dd->prec == HGRID_C64) lg_hip(hgrid_c128_to_c64(o, s, dd->n), "convert");
+    if (dd->prec != HGRID_C128) lg_hip(hgrid_c128_to_c64(o, s, dd->n), "convert");
     else lg_hip(hgrid_c64_to_c128(o, s, dd->n), "convert");
   } else {
     void *o = lg_c_out(dst);
@@ -392,7 +401,8 @@ REAL cgrid_integral_of_square(cgrid *grid) {
 REAL complex cgrid_integral_of_conjugate_product(cgrid *grida, cgrid *gridb) {
   void *a = lg_c_in_flushed(grida);
   void *b = lg_c_in_flushed(gridb);
-  if (grida->dev->prec != gridb->dev->prec) lg_fatal("cgrid_integral_of_conjugate_product: grids of different precision");
+  if (lg_prec_size(grida->dev->prec) != lg_prec_size(gridb->dev->prec))
+    lg_fatal("cgrid_integral_of_conjugate_product: grids of different storage precision");
   long n = (long)grida->dev->n;
   double *scr = lg_partials(hgrid_reduce_scratch(n));
   double *out = grida->dev->dscal + GS_TMP;

@@ -44,7 +44,8 @@ void lg_pass_init(hgrid_pass *p, const lg_shape *s, int prec, void *in, void *ou
   p->dir2 = +1;
   p->post = 1.0;
   p->mid = 1.0;
-  p->tw = lg_twiddles(s->n[axis], prec);
+  /* FP64 twiddles for FP64 arithmetic (double grids and FP32 storage with FP64 arithmetic) */
+  p->tw = lg_twiddles(s->n[axis], prec == HGRID_C64 ? HGRID_C64 : HGRID_C128);
 }
 
 void lg_run_pass(hgrid_pass *p) {

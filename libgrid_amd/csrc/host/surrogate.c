@@ -42,7 +42,7 @@ struct grid_surrogate {
   void *x0, *c1, *p1, *e2a, *c2, *p2, *e3a, *c3, *p3, *ba, *bb, *d3a, *d3, *d2a, *d2, *d1;
   /* gradients (bf16) */
   void *gc1, *gc2, *gc3, *gt1, *gt2;
-  float *vin, *tgt, *out, *partial, *scal;
+  float *vin, *tgt, *out, *partial, *bpartial, *scal;
   long partial_n;
   /* device-resident training set */
   float *set_v, *set_t;
@@ -154,6 +154,7 @@ grid_surrogate *grid_surrogate_alloc(INT side, INT batch, REAL vscale, INT seed)
   s->partial_n = pn;
   s->partial = (float *)dalloc(sizeof(float) * pn, "hipMalloc (surrogate partials)");
   s->scal = (float *)dalloc(sizeof(float) * 4, "hipMalloc (surrogate scalars)");
+  s->bpartial = (float *)dalloc(sizeof(float) * (size_t)(hgrid_nn_wgrad_chunks(nvb) * 192), "hipMalloc (bias partials)");
   return s;
 }
 
@@ -161,7 +162,7 @@ void grid_surrogate_free(grid_surrogate *s) {
   if (!s) return;
   void *bufs[] = {s->w,   s->g,   s->m1,  s->m2,  s->x0,  s->c1,  s->p1,  s->e2a, s->c2,  s->p2,  s->e3a,
                   s->c3,  s->p3,  s->ba,  s->bb,  s->d3a, s->d3,  s->d2a, s->d2,  s->d1,  s->gc1, s->gc2,
-                  s->gc3, s->gt1, s->gt2, s->vin, s->tgt, s->out, s->partial, s->scal, s->set_v, s->set_t};
+                  s->gc3, s->gt1, s->gt2, s->vin, s->tgt, s->out, s->partial, s->bpartial, s->scal, s->set_v, s->set_t};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
     if (bufs[i]) lg_hip(hgrid_free(bufs[i]), "hipFree");
   for (int l = 0; l < NLAYERS; ++l) {
@@ -228,10 +229,10 @@ static void forward(grid_surrogate *s, long nvb) {
 static void wgrad(grid_surrogate *s, int l, const void *dy, const void *x, int xs, int xo, int lev, long nvb) {
   const long nvl = lvl(s, nvb, lev);
   const int ci = kL[l].cin, co = kL[l].cout;
-  lg_hip(hgrid_nn_conv3_wgrad(dy, co, 0, co, x, xs, xo, ci, s->lgS - lev, nvl, s->partial), "nn_conv3_wgrad");
+  lg_hip(hgrid_nn_conv3_wgrad(dy, co, 0, co, x, xs, xo, ci, s->lgS - lev, nvl, s->partial, s->bpartial),
+         "nn_conv3_wgrad");
   lg_hip(hgrid_nn_reduce(s->partial, hgrid_nn_wgrad_partials(nvl), 27L * ci * co, s->g + s->off_w[l], 0), "nn_reduce");
-  lg_hip(hgrid_nn_colsum(dy, co, 0, co, nvl, s->partial), "nn_colsum");
-  lg_hip(hgrid_nn_reduce(s->partial, hgrid_nn_colsum_chunks(nvl), co, s->g + s->off_b[l], 0), "nn_reduce");
+  lg_hip(hgrid_nn_reduce(s->bpartial, hgrid_nn_wgrad_chunks(nvl), co, s->g + s->off_b[l], 0), "nn_reduce (bias)");
 }
 
 /* data gradient of layer l: dx = conv^T(dy), optionally times (act > 0) */

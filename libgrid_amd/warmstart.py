@@ -61,8 +61,9 @@ class Problem:
         else:
             self.lg.grid_wf_warm_start(self.w, self.r, net, GAUSS_WIDTH)
 
-    def relax(self, net, fp32_tol=None, tol=1e-10, max_steps=200000):
-        """Timed: start construction, optional FP32 relaxation to fp32_tol, FP64 to tol."""
+    def relax(self, net, fp32_tol=None, tol=1e-10, max_steps=200000, fp32_mode=b"\x01"):
+        """Timed: start construction, optional FP32-storage relaxation to fp32_tol (fp32_mode 1:
+        FP32 arithmetic, 2: FP64 arithmetic), then FP64 to tol."""
         lg = self.lg
         g = self.w.contents.grid
         e = C.c_double()
@@ -73,7 +74,7 @@ class Problem:
         t0 = time.perf_counter()
         s32 = 0
         if fp32_tol is not None:
-            lg.cgrid_set_precision(g, b"\x01")
+            lg.cgrid_set_precision(g, fp32_mode)
         self.start(net)
         if fp32_tol is not None:
             s32 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, fp32_tol, max_steps, C.byref(e))
@@ -152,11 +153,13 @@ def overlap(a, b):
     return float(np.sum(a * b) / np.sqrt(np.sum(a * a) * np.sum(b * b)))
 
 
-MODES = {  # name: (use the network, FP32 stage tolerance or None)
-    "gauss64": (False, None),
-    "net64": (True, None),
-    "gauss32": (False, 1e-8),
-    "net32": (True, 1e-8),
+MODES = {  # name: (use the network, FP32-storage stage tolerance or None, its arithmetic)
+    "gauss64": (False, None, None),
+    "net64": (True, None, None),
+    "gauss32": (False, 1e-8, b"\x01"),   # FP32 storage and arithmetic, then FP64
+    "net32": (True, 1e-8, b"\x01"),
+    "gauss32x": (False, 1e-10, b"\x02"),  # FP32 storage with FP64 arithmetic, then FP64 confirms
+    "net32x": (True, 1e-10, b"\x02"),
 }
 
 
@@ -172,8 +175,9 @@ def evaluate(lg, net, count, n=FINE, modes=("gauss64", "net32", "net64"), seed=H
         rec = dict(index=i, wells=len(prm[i]["depth"]))
         try:
             for m in modes:
-                use, t32 = MODES[m]
-                rec[m] = p.relax(net if use else None, fp32_tol=t32, tol=tol, max_steps=max_steps)
+                use, t32, mode32 = MODES[m]
+                rec[m] = p.relax(net if use else None, fp32_tol=t32, tol=tol, max_steps=max_steps,
+                                 fp32_mode=mode32 or b"\x01")
         finally:
             p.free()
         es = [rec[m]["energy"] for m in modes]
