@@ -79,6 +79,8 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
 #define HGRID_PF_POST_RED   0x0080u  /* post: partial sum of |psi|^2 of the stored values */
 #define HGRID_PF_NOSTORE    0x0100u  /* reduce only; do not write the array */
 #define HGRID_PF_PRE_VEFF   0x0200u  /* pre: rho_out = vext + g |psi|^2 (GPE potential) instead of |psi|^2 */
+#define HGRID_PF_PRE_AFTER  0x0800u  /* the PRE_RHO / PRE_VEFF / PRE_RED operations act on the transformed
+                                        data (a real-space state stored transformed along this axis) */
 #define HGRID_PF_KIN_REAL   0x0400u  /* with MID_KIN: the tables are real (imaginary time); the factor is
                                         applied as an FP64 real scale */
 

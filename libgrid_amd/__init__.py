@@ -177,8 +177,11 @@ PF_MID_GPE = 0x0040
 PF_POST_RED = 0x0080
 PF_NOSTORE = 0x0100
 PF_PRE_VEFF = 0x0200
+PF_KIN_REAL = 0x0400
+PF_PRE_AFTER = 0x0800
 HGRID_C128 = 0
 HGRID_C64 = 1
+HGRID_C64X = 2
 
 CG = C.POINTER(cgrid)
 RG = C.POINTER(rgrid)

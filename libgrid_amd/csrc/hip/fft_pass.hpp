@@ -222,28 +222,39 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = cmul(v[j], s);
     }
-    if (fl & (HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF)) {
-      const auto rrho = make_rsrc(a.rho_out, (unsigned long)ntot * 8);
-      const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
+  }
+  // density / V_eff / input reductions of the real-space data: before the transform, or after
+  // it with PRE_AFTER (the data is then stored transformed along this axis and the pass
+  // inverse-transforms it in registers); element j keeps flat index e0 + j * estep either way
+  auto pre_ops = [&]() {
+    const auto rrho = make_rsrc(a.rho_out, (unsigned long)ntot * 8);
+    const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const unsigned e = e0 + (unsigned)j * estep;
-        const double d = cnorm2(v[j]);
-        const double vx = (a.vext && active) ? bload_f64(rvx, e * 8u, 0) : 0.0;
-        if (active) {
-          if (fl & HGRID_PF_PRE_VEFF) bstore_f64(vx + a.g * d, rrho, e * 8u, 0);
-          else if (fl & HGRID_PF_PRE_RHO) bstore_f64(d, rrho, e * 8u, 0);
-        }
-        if (fl & HGRID_PF_PRE_RED) {
-          acc[HGRID_RED_NORM] += d;
-          acc[HGRID_RED_QUART] += d * d;
-          acc[HGRID_RED_VEXT] += vx * d;
-        }
+    for (int j = 0; j < R; ++j) {
+      const unsigned e = e0 + (unsigned)j * estep;
+      const double d = cnorm2(v[j]);
+      const double vx = (a.vext && active) ? bload_f64(rvx, e * 8u, 0) : 0.0;
+      if (active) {
+        if (fl & HGRID_PF_PRE_VEFF) bstore_f64(vx + a.g * d, rrho, e * 8u, 0);
+        else if (fl & HGRID_PF_PRE_RHO) bstore_f64(d, rrho, e * 8u, 0);
+      }
+      if (fl & HGRID_PF_PRE_RED) {
+        acc[HGRID_RED_NORM] += d;
+        acc[HGRID_RED_QUART] += d * d;
+        acc[HGRID_RED_VEXT] += vx * d;
       }
     }
+  };
+  constexpr unsigned PRE_ANY = HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF;
+  if constexpr (KIND == KIND_FFT) {
+    if ((fl & PRE_ANY) && !(fl & HGRID_PF_PRE_AFTER)) pre_ops();
   }
 
   fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
+
+  if constexpr (KIND == KIND_FFT) {
+    if ((fl & PRE_ANY) && (fl & HGRID_PF_PRE_AFTER)) pre_ops();
+  }
 
   if constexpr (KIND == KIND_FFT || KIND == KIND_KIN) {
     if (fl & HGRID_PF_MID_KRED) {

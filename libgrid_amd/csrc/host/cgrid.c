@@ -140,6 +140,7 @@ static void upload(cgrid *g) {
     free(tmp);
   }
   d->valid |= GV_DEV;
+  d->zpend = 0;
 }
 
 static void download(cgrid *g) {
@@ -158,9 +159,31 @@ static void download(cgrid *g) {
   d->valid |= GV_HOST;
 }
 
+/* Transform a state left along the contiguous axis by a fused step back to real space
+ * (unnormalised inverse: the step's kinetic factor already carried 1/N). */
+void lg_c_materialize(cgrid *g) {
+  grid_dev *d = g->dev;
+  if (!d->zpend) return;
+  lg_shape s;
+  lg_squeeze(g->nx, g->ny, g->nz, &s);
+  hgrid_pass p;
+  lg_pass_init(&p, &s, d->prec, d->d, d->d, 2);
+  p.dir1 = +1;
+  lg_run_pass(&p);
+  d->zpend = 0;
+}
+
+void *lg_c_raw(cgrid *g, const double **scale) {
+  grid_dev *d = lg_cdev(g);
+  if (!(d->valid & GV_DEV)) upload(g);
+  *scale = d->pending ? d->dscal + GS_SCALE : NULL;
+  return d->d;
+}
+
 void *lg_c_in(cgrid *g, const double **scale) {
   grid_dev *d = lg_cdev(g);
   if (!(d->valid & GV_DEV)) upload(g);
+  lg_c_materialize(g);
   if (scale) *scale = d->pending ? d->dscal + GS_SCALE : NULL;
   else if (d->pending) lg_c_flush(g);
   return d->d;
@@ -172,6 +195,7 @@ void *lg_c_inout(cgrid *g, const double **scale) { return lg_c_in(g, scale); }
 
 void *lg_c_out(cgrid *g) {
   grid_dev *d = lg_cdev(g);
+  d->zpend = 0; /* about to be overwritten */
   return d->d;
 }
 
@@ -180,6 +204,7 @@ void *lg_c_out(cgrid *g) {
 void lg_c_written(cgrid *g, int scale_consumed) {
   grid_dev *d = g->dev;
   d->valid = GV_DEV;
+  d->zpend = 0;
   if (scale_consumed && d->pending) {
     lg_hip(hgrid_scalar_set(d->dscal + GS_SCALE, 1.0, 0.0), "scalar_set");
     d->pending = 0;
@@ -206,6 +231,7 @@ void lg_c_flush(cgrid *g) {
 void lg_c_host_sync(cgrid *g) {
   grid_dev *d = g->dev;
   if (d->valid & GV_HOST) return;
+  lg_c_materialize(g);
   lg_c_flush(g);
   download(g);
 }
@@ -215,6 +241,7 @@ void cgrid_host_sync(cgrid *grid) { lg_c_host_sync(grid); }
 void cgrid_host_modified(cgrid *grid) {
   grid_dev *d = grid->dev;
   d->valid = GV_HOST;
+  d->zpend = 0;
   if (d->pending) {
     lg_hip(hgrid_scalar_set(d->dscal + GS_SCALE, 1.0, 0.0), "scalar_set");
     d->pending = 0;

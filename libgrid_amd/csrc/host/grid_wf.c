@@ -219,19 +219,32 @@ static void fused_step(wf *gwf, const step_args *sa) {
   lg_shape s;
   lg_squeeze(g->nx, g->ny, g->nz, &s);
   const double *scale;
-  void *psi = lg_c_inout(g, &scale);
+  void *psi = lg_c_raw(g, &scale);
   const int prec = d->prec, aK = 3 - s.m;
+  /* 3-D: the state stays transformed along the contiguous axis between steps (zpend); the
+   * step's first forward and last inverse transform along that axis cancel, and a read-only
+   * pass recovers the real-space quantities of the start-of-step state instead.  Only for
+   * FP64 arithmetic: with FP32 butterflies, imaginary-time relaxation from a transformed state
+   * needed about four times the steps to meet the stopping rule, so FP32 grids keep the
+   * real-space state between steps. */
+  const int defer = s.m == 3 && prec != HGRID_C64;
+  if (d->zpend && !defer) lg_c_materialize(g);
+  const int zin = d->zpend;
   ktab_cache *kt = lg_kinetic_tables(g, sa->tm_re, sa->tm_im, g->fft_norm);
   ktab_cache *k2 = (sa->want_energy) ? lg_k2_tables(g) : NULL;
   if (sa->gpe && !d->rho) lg_hip(hgrid_malloc((void **)&d->rho, d->n * sizeof(double)), "hipMalloc (density)");
 
   const long nb2 = hgrid_pass_blocks(prec, s.n[0], s.n[1], s.n[2], 2);
   const long nbK = hgrid_pass_blocks(prec, s.n[0], s.n[1], s.n[2], aK);
-  double *part = lg_partials(HGRID_RED_NCHAN * (2 * nb2 + nbK));
+  const long nb1 = s.m == 3 ? hgrid_pass_blocks(prec, s.n[0], s.n[1], s.n[2], 1) : 0;
+  const long nb9 = nb1 > nb2 ? nb1 : nb2; /* the output norm comes from the last y pass when deferred */
+  double *part = lg_partials(HGRID_RED_NCHAN * (nb2 + nbK + nb9));
   double *part1 = part, *partK = part + HGRID_RED_NCHAN * nb2, *part9 = partK + HGRID_RED_NCHAN * nbK;
   hgrid_pass p;
 
-  /* P1: forward along the contiguous axis; pending scale, density and input reductions */
+  /* P1: forward along the contiguous axis; pending scale, density and input reductions.  With
+   * the state already transformed (zin) the pass instead inverse-transforms in registers for
+   * the density / reductions and stores nothing; the first y pass then applies the scale. */
   lg_pass_init(&p, &s, prec, psi, psi, 2);
   if (scale) { p.flags |= HGRID_PF_PRESCALE; p.scale = scale; }
   if (sa->gpe) {
@@ -242,7 +255,18 @@ static void fused_step(wf *gwf, const step_args *sa) {
     p.g = sa->g;
   }
   if (sa->want_energy) { p.flags |= HGRID_PF_PRE_RED; p.vext = sa->vext; p.partials = part1; }
-  lg_run_pass(&p);
+  if (zin) {
+    p.dir1 = +1;
+    p.flags |= HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE;
+    if (sa->gpe || sa->want_energy) lg_run_pass(&p);
+  } else {
+    lg_run_pass(&p);
+  }
+  /* with zin the first y pass reads the stored transform S directly.  The forward transform
+   * of the real-space state IDFT(S) it replaces would have given n2 * S (an unnormalised round
+   * trip), which the kinetic factors' 1/N^2 accounts for: that pass applies n2 and the scale */
+  const double *yscale = zin ? scale : NULL;
+  int ypost = zin;
   /* all reductions of the step and the pending-scale update go into one finalize launch */
   hgrid_finalize fin;
   memset(&fin, 0, sizeof(fin));
@@ -254,6 +278,15 @@ static void fused_step(wf *gwf, const step_args *sa) {
   for (int half = 0; half < 2; ++half) {
     for (int axis = 1; axis > aK; --axis) {
       lg_pass_init(&p, &s, prec, psi, psi, axis);
+      if (yscale) {
+        p.flags |= HGRID_PF_PRESCALE;
+        p.scale = yscale;
+        yscale = NULL;
+      }
+      if (ypost) {
+        p.post = (double)s.n[2];
+        ypost = 0;
+      }
       lg_run_pass(&p);
     }
     /* kinetic: forward, K / N, inverse along the outermost axis */
@@ -277,6 +310,12 @@ static void fused_step(wf *gwf, const step_args *sa) {
     for (int axis = aK + 1; axis <= 1; ++axis) {
       lg_pass_init(&p, &s, prec, psi, psi, axis);
       p.dir1 = +1;
+      if (defer && half == 1 && axis == 1 && sa->want_post_norm) {
+        /* sum |S|^2 of the output S, still transformed along the contiguous axis: the state
+         * is IDFT(S), whose sum is n2 times larger (Parseval); the finalize multiplies it in */
+        p.flags |= HGRID_PF_POST_RED;
+        p.partials = part9;
+      }
       lg_run_pass(&p);
     }
     if (half == 0) {
@@ -309,26 +348,32 @@ static void fused_step(wf *gwf, const step_args *sa) {
       }
     }
   }
-  /* last: inverse along the contiguous axis (1/N already applied with K) */
-  lg_pass_init(&p, &s, prec, psi, psi, 2);
-  p.dir1 = +1;
-  if (sa->want_post_norm) { p.flags |= HGRID_PF_POST_RED; p.partials = part9; }
-  lg_run_pass(&p);
+  /* last: inverse along the contiguous axis (1/N already applied with K), unless deferred */
+  long nbpost = nb2;
+  if (!defer) {
+    lg_pass_init(&p, &s, prec, psi, psi, 2);
+    p.dir1 = +1;
+    if (sa->want_post_norm) { p.flags |= HGRID_PF_POST_RED; p.partials = part9; }
+    lg_run_pass(&p);
+  } else {
+    nbpost = nb1;
+  }
   if (sa->want_post_norm) {
-    hgrid_reduce_job j = {part9, nb2, HGRID_RED_POST, 1, d->dscal + GS_RED + HGRID_RED_POST};
+    hgrid_reduce_job j = {part9, nbpost, HGRID_RED_POST, 1, d->dscal + GS_RED + HGRID_RED_POST};
     fin.job[fin.njobs++] = j;
   }
   if (sa->renorm_target > 0.0) {
     fin.scale = d->dscal + GS_SCALE;
     fin.norm_sum = d->dscal + GS_RED + HGRID_RED_POST;
     fin.target = sa->renorm_target;
-    fin.dv = lg_step_factor(g->nx, g->ny, g->nz, g->step);
+    fin.dv = lg_step_factor(g->nx, g->ny, g->nz, g->step) * (defer ? (double)s.n[2] : 1.0);
   } else if (scale) {
     fin.scale = d->dscal + GS_SCALE; /* the pending scale was applied by the first pass */
   }
   if (fin.njobs || fin.scale) lg_hip(hgrid_step_finalize(&fin), "step_finalize");
   d->valid = GV_DEV;
   d->pending = sa->renorm_target > 0.0;
+  d->zpend = defer;
 }
 
 /* unfused sequence for grids with fewer than two non-trivial axes (reference order) */

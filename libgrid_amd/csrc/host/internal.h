@@ -36,6 +36,9 @@ typedef struct grid_dev {
   void *d;                 /* device array, or NULL */
   int valid;               /* GV_HOST | GV_DEV */
   int pending;             /* device scale slot differs from 1 */
+  int zpend;               /* the device array holds the unnormalised forward transform of the
+                              state along the squeezed contiguous axis (fused steps leave it so
+                              between steps); accessors transform back before other use */
   double *dscal;           /* GS_N device doubles */
   /* wavefunction helpers (complex grids used as wf->grid) */
   void *work;              /* device-only scratch of the same size (kinetic energy etc.) */
@@ -63,6 +66,8 @@ void *lg_c_inout(cgrid *g, const double **scale);
 void lg_c_written(cgrid *g, int scale_consumed);
 void lg_c_set_pending(cgrid *g, double re, double im); /* multiply the pending scale */
 void lg_c_flush(cgrid *g);
+void *lg_c_raw(cgrid *g, const double **scale);  /* device data as stored (zpend kept), for the fused step */
+void lg_c_materialize(cgrid *g);                 /* undo zpend: inverse transform along the contiguous axis */
 void lg_c_host_sync(cgrid *g);
 grid_dev *lg_rdev(rgrid *g);
 const double *lg_r_in(rgrid *g);
