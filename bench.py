@@ -250,6 +250,8 @@ def batch2d_bench(lib, LG, count=256, tol=1e-8, dt=0.1):
     t0 = time.perf_counter()
     secs = lib.grid_wf_batch2d_relax(count, warr, parr, dt, tol, 200000, e, st)
     wall = time.perf_counter() - t0
+    p2n, p2s = C.c_long(), C.c_double()
+    lib.grid_batch2d_last_phase2(C.byref(p2n), C.byref(p2s))
     ts, te = (C.c_double * count)(), (C.c_double * count)()
     lib.grid_batch2d_solve_times(ts, te)
     busy = float(sum(te[i] - ts[i] for i in range(count)))
@@ -261,9 +263,11 @@ def batch2d_bench(lib, LG, count=256, tol=1e-8, dt=0.1):
         lib.rgrid_free(r)
     out = dict(workload=f"{count} x 2-D wells (n in 64..256, spacing 0.2, dt {dt}, tol {tol})", solves_per_s=count / secs,
                launch_s=secs, wall_s_incl_packing=wall, steps_median=int(np.median(steps)), steps_max=int(steps.max()),
-               utilisation=busy / (cus * secs) if cus > 0 else None, autotune=report)
-    log(f"batch2d: {count / secs:.1f} solves/s ({secs:.3f} s launch), steps median {out['steps_median']} max "
-        f"{out['steps_max']}, utilisation {out['utilisation']:.3f}")
+               persistent_launch_s=secs - p2s.value, straggler_solves=int(p2n.value), straggler_s=p2s.value,
+               utilisation_persistent=busy / (cus * (secs - p2s.value)) if cus > 0 else None, autotune=report)
+    log(f"batch2d: {count / secs:.1f} solves/s ({secs - p2s.value:.3f} s persistent launch + {p2n.value} stragglers "
+        f"{p2s.value:.3f} s), steps median {out['steps_median']} max {out['steps_max']}, launch utilisation "
+        f"{out['utilisation_persistent']:.3f}")
     return out
 
 

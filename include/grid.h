@@ -264,10 +264,16 @@ INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_s
  * steps of length dt, renormalised every step, until its FFT energy is converged to tol by the
  * rule of grid_wf_gpe_relax (or max_steps).  The normalised states are written back to gwf[i]->grid; energy[i]
  * and steps[i] (may be NULL) receive the final energy and the step at which tol was met
- * (max_steps if never).  Returns the GPU time of the launch in seconds.  The first call on a
+ * (max_steps if never).  Returns the GPU time of the launch plus the wall time of the
+ * straggler phase (grid_batch2d_set_straggler_cap) in seconds.  The first call on a
  * device runs grid_batch2d_autotune unless a tuned tile is cached. */
 REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, REAL tol, INT max_steps, REAL *energy,
                            INT *steps);
+/* Solves still unconverged after this many steps of the persistent launch (default 3000; 0 =
+ * no cap) are continued one at a time on the multi-kernel FP32 path, which spreads one grid over
+ * the whole GPU; grid_batch2d_last_phase2 reports how many and their wall time. */
+void grid_batch2d_set_straggler_cap(INT steps);
+void grid_batch2d_last_phase2(INT *count, REAL *seconds);
 /* Time every tile configuration of the batched kernel on these problems for probe_steps steps,
  * keep the fastest (also cached in $XDG_CACHE_HOME or ~/.cache/libgrid_amd/batch2d_tile.txt)
  * and return a one-line report. */

@@ -265,7 +265,11 @@ int hgrid_nn_head_bwd(const void *d, const float *wh, const float *out, const fl
 long hgrid_nn_head_blocks(long nvox);
 /* BF16 weights wb[co][t][ci] and reflected-transposed wt[ci][26 - t][co] from FP32 w */
 int hgrid_nn_wprep(const float *w, int cout, int cin, void *wb, void *wt);
-int hgrid_nn_adam(float *w, const float *g, float *m1, float *m2, long n, float lr, float b1, float b2, float eps, int t);
+/* Adam step; the gradient is multiplied by gscale[0] (may be NULL), and 0 there skips the step */
+int hgrid_nn_adam(float *w, const float *g, float *m1, float *m2, long n, float lr, float b1, float b2, float eps, int t,
+                  const float *gscale);
+/* factor[0] = min(1, clip / |g|) (0 if |g| is not finite), factor[1] = |g|; scratch: 512 doubles */
+int hgrid_nn_clip_factor(const float *g, long n, double clip, double *scratch, float *factor);
 /* input tensor [m][8]: vscale * v[m], then zeros */
 int hgrid_nn_input(const float *v, float vscale, void *x0, long nvox);
 /* one S^3 sample under a symmetry of the periodic cube (axis permutation perm < 6, reflections

@@ -420,11 +420,47 @@ __global__ void k_wprep(const float *__restrict__ w, int cout, int cin, bf16 *__
   wt[((long)ci * 27 + (26 - t)) * cout + co] = v;
 }
 
+// global gradient norm for clipping: per-block sums of squares (FP64), then one workgroup
+// turns them into the factor min(1, clip / |g|) (0 when |g| is not finite: the step is skipped)
+__global__ void __launch_bounds__(256) k_sumsq(const float *__restrict__ g, long n, double *__restrict__ partial) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) s += (double)g[e] * g[e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_clip_factor(const double *__restrict__ partial, int nb, double clip,
+                                                     float *__restrict__ factor) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += partial[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(red[0]);
+    factor[0] = (float)(isfinite(nrm) ? (nrm > clip ? clip / nrm : 1.0) : 0.0);
+    factor[1] = (float)nrm;
+  }
+}
+
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ gr, float *__restrict__ m1,
-                       float *__restrict__ m2, long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+                       float *__restrict__ m2, long n, float lr, float b1, float b2, float eps, float c1, float c2,
+                       const float *__restrict__ gscale) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  const float gg = gr[e];
+  const float f = gscale ? gscale[0] : 1.f;
+  if (f == 0.f) return;  // non-finite gradient: no update
+  const float gg = gr[e] * f;
   const float a = b1 * m1[e] + (1.f - b1) * gg;
   const float s = b2 * m2[e] + (1.f - b2) * gg * gg;
   m1[e] = a;
@@ -620,10 +656,18 @@ extern "C" int hgrid_nn_wprep(const float *w, int cout, int cin, void *wb, void 
   return lg::check_launch("nn_wprep");
 }
 
+extern "C" int hgrid_nn_clip_factor(const float *g, long n, double clip, double *scratch, float *factor) {
+  const int nb = 512;
+  hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, lg::stream(), g, n, scratch);
+  hipLaunchKernelGGL(k_clip_factor, dim3(1), dim3(256), 0, lg::stream(), scratch, nb, clip, factor);
+  return lg::check_launch("nn_clip_factor");
+}
+
 extern "C" int hgrid_nn_adam(float *w, const float *g, float *m1, float *m2, long n, float lr, float b1, float b2,
-                             float eps, int t) {
+                             float eps, int t, const float *gscale) {
   const float c1 = 1.f / (1.f - powf(b1, (float)t)), c2 = 1.f / (1.f - powf(b2, (float)t));
-  hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, lg::stream(), w, g, m1, m2, n, lr, b1, b2, eps, c1, c2);
+  hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, lg::stream(), w, g, m1, m2, n, lr, b1, b2, eps, c1, c2,
+                     gscale);
   return lg::check_launch("nn_adam");
 }
 

@@ -20,6 +20,9 @@ static const b2d_tile kCandidates[] = {{0, 256, 1}, {0, 256, 2}, {0, 512, 1}, {1
 #define NCAND ((int)(sizeof(kCandidates) / sizeof(kCandidates[0])))
 
 static b2d_tile g_tile = {-1, 0, 0};
+static INT g_cap = 3000;            /* persistent-launch step cap (0: none) */
+static INT g_phase2_count = 0;      /* solves continued on the multi-kernel path by the last call */
+static double g_phase2_seconds = 0.0;
 static char g_report[2048];
 
 static const char *cache_path(char *buf, size_t len) {
@@ -322,8 +325,14 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
     /* first use on this device: tune on (up to) the first 64 problems for 20 steps */
     grid_batch2d_autotune(nsolves < 64 ? nsolves : 64, gwf, potential, dt, 20);
   }
+  /* Phase 1: the persistent launch, every solve capped at g_cap steps.  A solve that has not
+   * converged by then is a nearly degenerate problem with a long imaginary-time tail; in one
+   * workgroup it would hold the whole launch (the makespan of a 256-problem batch was a single
+   * 22 000-step solve at 3.5% utilisation).  Phase 2 continues those few on the multi-kernel
+   * FP32 path, which puts the whole GPU on one grid. */
+  const INT cap = (g_cap > 0 && g_cap < max_steps) ? g_cap : max_steps;
   b2d_batch b;
-  b2d_pack(&b, nsolves, gwf, potential, dt, tol, max_steps);
+  b2d_pack(&b, nsolves, gwf, potential, dt, tol, cap);
   const double ms = b2d_run(&b, &g_tile);
   double *out = (double *)malloc(sizeof(double) * HGRID_B2D_NOUT * nsolves);
   lg_hip(hgrid_memcpy_d2h(out, b.out, sizeof(double) * HGRID_B2D_NOUT * nsolves), "hipMemcpy (batch out)");
@@ -345,8 +354,33 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
     if (energy) energy[i] = out[HGRID_B2D_NOUT * i];
     if (steps) steps[i] = (INT)out[HGRID_B2D_NOUT * i + 2];
   }
-  free(out);
-  free(hpsi);
   b2d_free(&b);
-  return ms * 1e-3;
+  free(hpsi);
+  /* phase 2 */
+  grid_timer tm;
+  grid_timer_start(&tm);
+  g_phase2_count = 0;
+  for (INT i = 0; i < nsolves && cap < max_steps; ++i) {
+    if ((INT)out[HGRID_B2D_NOUT * i + 2] < cap) continue;
+    cgrid *g = gwf[i]->grid;
+    const char prec0 = cgrid_get_precision(g);
+    cgrid_set_precision(g, GRID_PREC_SINGLE);
+    REAL e2 = 0.0;
+    const INT s2 = grid_wf_gpe_relax(gwf[i], potential[i], 0.0, dt, tol, max_steps - cap, &e2);
+    cgrid_set_precision(g, prec0);
+    if (energy) energy[i] = e2;
+    if (steps) steps[i] = cap + s2;
+    ++g_phase2_count;
+  }
+  grid_hip_sync();
+  g_phase2_seconds = g_phase2_count ? grid_timer_wall_clock_time(&tm) : 0.0;
+  free(out);
+  return ms * 1e-3 + g_phase2_seconds;
+}
+
+void grid_batch2d_set_straggler_cap(INT steps) { g_cap = steps; }
+
+void grid_batch2d_last_phase2(INT *count, REAL *seconds) {
+  if (count) *count = g_phase2_count;
+  if (seconds) *seconds = g_phase2_seconds;
 }

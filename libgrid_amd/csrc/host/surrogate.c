@@ -43,6 +43,7 @@ struct grid_surrogate {
   /* gradients (bf16) */
   void *gc1, *gc2, *gc3, *gt1, *gt2;
   float *vin, *tgt, *out, *partial, *bpartial, *scal;
+  double *clip_scratch;
   long partial_n;
   /* device-resident training set */
   float *set_v, *set_t;
@@ -154,6 +155,7 @@ grid_surrogate *grid_surrogate_alloc(INT side, INT batch, REAL vscale, INT seed)
   s->partial_n = pn;
   s->partial = (float *)dalloc(sizeof(float) * pn, "hipMalloc (surrogate partials)");
   s->scal = (float *)dalloc(sizeof(float) * 4, "hipMalloc (surrogate scalars)");
+  s->clip_scratch = (double *)dalloc(sizeof(double) * 512, "hipMalloc (clip scratch)");
   s->bpartial = (float *)dalloc(sizeof(float) * (size_t)(hgrid_nn_wgrad_chunks(nvb) * 192), "hipMalloc (bias partials)");
   return s;
 }
@@ -162,7 +164,7 @@ void grid_surrogate_free(grid_surrogate *s) {
   if (!s) return;
   void *bufs[] = {s->w,   s->g,   s->m1,  s->m2,  s->x0,  s->c1,  s->p1,  s->e2a, s->c2,  s->p2,  s->e3a,
                   s->c3,  s->p3,  s->ba,  s->bb,  s->d3a, s->d3,  s->d2a, s->d2,  s->d1,  s->gc1, s->gc2,
-                  s->gc3, s->gt1, s->gt2, s->vin, s->tgt, s->out, s->partial, s->bpartial, s->scal, s->set_v, s->set_t};
+                  s->gc3, s->gt1, s->gt2, s->vin, s->tgt, s->out, s->partial, s->bpartial, s->scal, s->set_v, s->set_t, s->clip_scratch};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
     if (bufs[i]) lg_hip(hgrid_free(bufs[i]), "hipFree");
   for (int l = 0; l < NLAYERS; ++l) {
@@ -292,9 +294,13 @@ static double backward(grid_surrogate *s, long nb) {
   return (double)lsum / (double)nvb;
 }
 
+/* Adam with the gradient clipped to global norm 1: a single bad minibatch (a loss spike) can
+ * otherwise push every ReLU of a layer negative at once, after which the network stays dead */
 static void adam_step(grid_surrogate *s, double lr) {
   ++s->adam_t;
-  lg_hip(hgrid_nn_adam(s->w, s->g, s->m1, s->m2, s->np, (float)lr, 0.9f, 0.999f, 1e-8f, s->adam_t), "nn_adam");
+  lg_hip(hgrid_nn_clip_factor(s->g, s->np, 1.0, s->clip_scratch, s->scal + 2), "nn_clip_factor");
+  lg_hip(hgrid_nn_adam(s->w, s->g, s->m1, s->m2, s->np, (float)lr, 0.9f, 0.999f, 1e-8f, s->adam_t, s->scal + 2),
+         "nn_adam");
   refresh_weights(s);
 }
 
@@ -413,8 +419,10 @@ REAL grid_surrogate_train_ex(grid_surrogate *s, INT epochs, REAL lr, INT seed, i
     last = sum / (double)s->set_n;
     if (loss_history) loss_history[ep] = last;
     if (flags & GRID_SURROGATE_PROGRESS) {
-      fprintf(stderr, "libgrid: surrogate epoch %ld/%ld loss %.6f (%.1f s)\n", (long)ep + 1, (long)epochs, last,
-              grid_timer_wall_clock_time(&tm));
+      float gn[2];
+      lg_hip(hgrid_memcpy_d2h(gn, s->scal + 2, sizeof(gn)), "hipMemcpy (gradient norm)");
+      fprintf(stderr, "libgrid: surrogate epoch %ld/%ld loss %.6f, last |g| %.3g (%.1f s)\n", (long)ep + 1, (long)epochs,
+              last, (double)gn[1], grid_timer_wall_clock_time(&tm));
       fflush(stderr);
     }
   }

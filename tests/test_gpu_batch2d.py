@@ -110,6 +110,9 @@ def test_batch2d_256_solves(lg):
     conv = int(np.sum(st < MAX_STEPS))
     print(f"batched: {ns} solves in {secs:.3f} s GPU ({wall:.3f} s wall incl. packing) = {ns / secs:.1f} solves/s; "
           f"converged {conv}/{ns}; steps min/median/max {st.min()}/{int(np.median(st))}/{st.max()}")
+    p2n, p2s = C.c_long(), C.c_double()
+    lg.grid_batch2d_last_phase2(C.byref(p2n), C.byref(p2s))
+    print(f"straggler phase: {p2n.value} solves continued on the multi-kernel FP32 path in {p2s.value:.3f} s")
     t_s = (C.c_double * ns)()
     t_e = (C.c_double * ns)()
     assert lg.grid_batch2d_solve_times(t_s, t_e) == ns
