@@ -256,6 +256,9 @@ REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g);
  * D2 q / (1 - q), q = D2 / D1, of the energy still to come; a rise smaller than tol (rounding)
  * also ends it.  Returns the number of steps taken and stores the last energy in *energy. */
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_steps, REAL *energy);
+/* Relaxation replays batches of 16 steps as a HIP graph after the first batch (default on);
+ * grid_hip_graphs(0) launches every step eagerly instead. */
+void grid_hip_graphs(int enable);
 
 /* ---- batched 2-D ground states (this build's addition) ----------------------------------
  * Relax nsolves independent 1 x n x n wavefunctions (n in {64, 96, 128, 192, 256}) in

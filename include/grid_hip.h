@@ -192,6 +192,13 @@ int hgrid_scalar_renorm(double *s, const double *sum, double target, double dv);
 int hgrid_fd_laplace_expect(double *out, const void *psi, int prec, long nx, long ny, long nz, int bc,
                             const double *scale, double *scratch);
 
+/* HIP graphs of the library stream (no memset nodes are ever recorded: the library clears
+ * device memory with kernels) */
+int hgrid_graph_begin(void);
+int hgrid_graph_end(void **exec);
+int hgrid_graph_launch(void *exec);
+int hgrid_graph_free(void *exec);
+
 /* ---- batched 2-D imaginary-time solves ---------------------------------------------------- */
 
 /* One n x n solve of the persistent batched kernel (complex64 storage, FP64 reductions). */

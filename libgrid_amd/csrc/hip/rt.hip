@@ -92,6 +92,30 @@ extern "C" int hgrid_memcpy_d2d(void *dst, const void *src, size_t bytes) {
 
 extern "C" int hgrid_sync(void) { return check(hipStreamSynchronize(lg::g_stream), "hipStreamSynchronize"); }
 
+// HIP graphs of the library stream: capture a sequence of library calls once, replay it.
+extern "C" int hgrid_graph_begin(void) {
+  return check(hipStreamBeginCapture(lg::g_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+}
+
+extern "C" int hgrid_graph_end(void **exec) {
+  hipGraph_t graph = nullptr;
+  if (check(hipStreamEndCapture(lg::g_stream, &graph), "hipStreamEndCapture")) return -1;
+  hipGraphExec_t ge = nullptr;
+  const int rc = check(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(graph);
+  if (rc) return -1;
+  *exec = (void *)ge;
+  return 0;
+}
+
+extern "C" int hgrid_graph_launch(void *exec) {
+  return check(hipGraphLaunch((hipGraphExec_t)exec, lg::g_stream), "hipGraphLaunch");
+}
+
+extern "C" int hgrid_graph_free(void *exec) {
+  return exec ? check(hipGraphExecDestroy((hipGraphExec_t)exec), "hipGraphExecDestroy") : 0;
+}
+
 extern "C" int hgrid_event_create(void **ev) {
   hipEvent_t e;
   if (check(hipEventCreate(&e), "hipEventCreate")) return -1;

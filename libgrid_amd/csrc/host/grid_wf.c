@@ -579,6 +579,10 @@ REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g_nl) {
   return t + v + 0.5 * g_nl * q * dv;
 }
 
+static int g_use_graphs = 1;
+
+void grid_hip_graphs(int enable) { g_use_graphs = enable ? 1 : 0; }
+
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy) {
   grid_dev *d = gwf->grid->dev;
   const INT batch = 16;
@@ -590,12 +594,26 @@ INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
   INT done = -1, n = 0;
   lg_relax_state rs;
   lg_relax_init(&rs);
+  /* After the first batch every step is the same sequence of launches on the same buffers
+   * (tables cached, scale pending, 3-D state deferred), so one batch of 16 steps is captured
+   * as a HIP graph and replayed: small grids are launch-bound otherwise. */
+  void *graph = NULL;
   while (n < max_steps && done < 0) {
     INT nb = (max_steps - n) < batch ? (max_steps - n) : batch;
-    for (INT b = 0; b < nb; ++b) {
-      grid_wf_gpe_propagate(gwf, vext, g_nl, -I * dt, 1);
-      lg_hip(hgrid_memcpy_d2d(hist + HGRID_RED_NCHAN * b, d->dscal + GS_RED, sizeof(double) * HGRID_RED_NCHAN),
-             "hipMemcpy D2D (energy history)");
+    if (graph && nb == batch) {
+      lg_hip(hgrid_graph_launch(graph), "hipGraphLaunch (relaxation)");
+    } else {
+      const int capture = g_use_graphs && !graph && n > 0 && nb == batch;
+      if (capture) lg_hip(hgrid_graph_begin(), "hipStreamBeginCapture (relaxation)");
+      for (INT b = 0; b < nb; ++b) {
+        grid_wf_gpe_propagate(gwf, vext, g_nl, -I * dt, 1);
+        lg_hip(hgrid_memcpy_d2d(hist + HGRID_RED_NCHAN * b, d->dscal + GS_RED, sizeof(double) * HGRID_RED_NCHAN),
+               "hipMemcpy D2D (energy history)");
+      }
+      if (capture) {
+        lg_hip(hgrid_graph_end(&graph), "hipStreamEndCapture (relaxation)");
+        lg_hip(hgrid_graph_launch(graph), "hipGraphLaunch (relaxation)"); /* capture does not execute */
+      }
     }
     lg_hip(hgrid_memcpy_d2h(host, hist, sizeof(double) * HGRID_RED_NCHAN * nb), "hipMemcpy D2H (energy history)");
     for (INT b = 0; b < nb; ++b) {
@@ -608,6 +626,7 @@ INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
     }
     n += nb;
   }
+  if (graph) lg_hip(hgrid_graph_free(graph), "hipGraphExecDestroy");
   lg_hip(hgrid_free(hist), "hipFree");
   if (energy) *energy = e;
   return done < 0 ? n : done;

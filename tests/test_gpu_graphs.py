@@ -1,0 +1,58 @@
+"""Imaginary-time relaxation replayed as HIP graphs against the same relaxation launched step
+by step: the kernels and their arguments are the same, so energies, step counts and states must
+agree exactly.  Every replay is followed by the host's own D2H copy and stopping-rule check, so
+replays alternate with eager work as they do in use."""
+import ctypes as C
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import libgrid_amd as LG
+from libgrid_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def relax(lg, shape, step, psi, v, prec, graphs, tol=1e-10, max_steps=20000):
+    nx, ny, nz = shape
+    w = lg.grid_wf_alloc(nx, ny, nz, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"g")
+    LG.set_cgrid(w.contents.grid, psi)
+    lg.cgrid_set_precision(w.contents.grid, prec)
+    r = lg.rgrid_alloc(nx, ny, nz, step, LG.boundary("rgrid_value_outside_periodic"), None, b"v")
+    LG.set_rgrid(r, v)
+    lg.grid_hip_graphs(1 if graphs else 0)
+    e = C.c_double()
+    lg.grid_hip_sync()
+    t0 = time.perf_counter()
+    steps = lg.grid_wf_gpe_relax(w, r, 0.05, 0.1, tol, max_steps, C.byref(e))
+    lg.grid_hip_sync()
+    sec = time.perf_counter() - t0
+    out = LG.get_cgrid(w.contents.grid)
+    lg.grid_wf_free(w)
+    lg.rgrid_free(r)
+    lg.grid_hip_graphs(1)
+    return e.value, steps, out, sec
+
+
+@pytest.mark.parametrize("shape,prec", [((1, 128, 128), b"\x01"), ((32, 32, 32), b"\x00"), ((64, 64, 64), b"\x01")])
+def test_graph_replay_equals_eager(lg, shape, prec):
+    case(f"graph replay vs eager {shape} prec {prec[0]}")
+    rng = np.random.default_rng(3)
+    step = 0.3
+    x = [W.axes(n, step) for n in shape]
+    r2 = x[0][:, None, None] ** 2 + x[1][None, :, None] ** 2 + x[2][None, None, :] ** 2
+    v = 0.02 * r2 - 0.3 * np.exp(-r2 / 4.0)
+    psi = (np.exp(-r2 / 8.0) * (1 + 0.1 * rng.normal(size=shape))).astype(np.complex128)
+    ge, gs, gpsi, gsec = relax(lg, shape, step, psi, v, prec, True)
+    ee, es, epsi, esec = relax(lg, shape, step, psi, v, prec, False)
+    print(f"graphs: {gs} steps {gsec:.3f} s E {ge:.15f}; eager: {es} steps {esec:.3f} s E {ee:.15f}")
+    assert gs == es
+    assert ge == ee
+    np.testing.assert_array_equal(gpsi, epsi)
