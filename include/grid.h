@@ -60,7 +60,9 @@
 #define WF_PERIODIC_BOUNDARY 3
 
 /* Wavefunction propagators (reference src/defs.h:26-32).  This build implements
- * WF_2ND_ORDER_FFT; grid_wf_alloc accepts the others and grid_wf_propagate aborts for them. */
+ * WF_2ND_ORDER_FFT and WF_2ND_ORDER_CFFT (the same step with components |k - k0| >= kmax
+ * removed in the kinetic phase); grid_wf_alloc accepts the others and grid_wf_propagate aborts
+ * for them. */
 #define WF_2ND_ORDER_FFT 1
 #define WF_4TH_ORDER_FFT 2
 #define WF_2ND_ORDER_CFFT 3
@@ -223,6 +225,7 @@ void grid_wf_free(wf *gwf);                                                     
 void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time);          /* :379 */
 void grid_wf_propagate_potential(wf *gwf, REAL complex tstep, cgrid *potential, REAL cons); /* :569 */
 void grid_wf_propagate_kinetic_fft(wf *gwf, REAL complex time);                /* grid_wf_fft.c:141 */
+void grid_wf_propagate_kinetic_cfft(wf *gwf, REAL complex time);               /* grid_wf_fft.c:206 */
 void grid_wf_density(wf *gwf, rgrid *density);                                  /* :611 */
 void grid_wf_zero(wf *gwf);
 void grid_wf_constant(wf *gwf, REAL complex c);

@@ -83,6 +83,8 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
                                         data (a real-space state stored transformed along this axis) */
 #define HGRID_PF_KIN_REAL   0x0400u  /* with MID_KIN: the tables are real (imaginary time); the factor is
                                         applied as an FP64 real scale */
+#define HGRID_PF_KCUT       0x1000u  /* with MID_KIN: zero psi where (kc_a[n] + kc_1[c1]) + kc_2[c2] >= kcut2
+                                        (the k-space cutoff of the CFFT propagator) */
 
 /* Reduction channels (partials[chan * nblocks + block]). */
 #define HGRID_RED_NORM   0   /* sum |psi|^2 of the pass input   (PRE_RED) */
@@ -109,6 +111,8 @@ typedef struct hgrid_pass {
    * other squeezed coordinates c1, c2 of the pencil (see hgrid_pass_coords) */
   const void *ktab_a, *ktab_1, *ktab_2;
   const double *k2_a, *k2_1, *k2_2;   /* |k|^2 per axis for MID_KRED (no k0 shift) */
+  const double *kc_a, *kc_1, *kc_2;   /* (k - k0)^2 per axis for KCUT */
+  double kcut2;                       /* KCUT threshold kmax^2 */
   /* potential */
   const void *pot;       /* complex potential, same layout and precision as in */
   const double *vext;    /* dense real external potential (FP64) */
@@ -171,9 +175,11 @@ int hgrid_sum_real(double *out, const double *a, long n, int squared, double *sc
 int hgrid_potential_phase(void *psi, const void *pot, int prec, long n, double c_re, double c_im, double cons,
                           const double *scale);
 /* psi[i,j,k] *= ta[i] * tb[j] * tc[k]  (3 tables of double2 factors for either precision;
- * the factor is formed in FP64 and rounded once onto psi) */
+ * the factor is formed in FP64 and rounded once onto psi).  With kcx != NULL, psi[i,j,k] = 0
+ * where (kcx[i] + kcy[j]) + kcz[k] >= kcut2 (the CFFT cutoff; kc* hold (k - k0)^2). */
 int hgrid_kinetic_phase(void *psi, int prec, long nx, long ny, long nz, const void *ta, const void *tb,
-                        const void *tc, const double *scale);
+                        const void *tc, const double *scale, const double *kcx, const double *kcy,
+                        const double *kcz, double kcut2);
 /* psi *= (c_re + i c_im) * (*scale if scale) */
 int hgrid_cscale(void *psi, int prec, long n, double c_re, double c_im, const double *scale);
 /* dst = |src|^2 (dense FP64 output) */

@@ -247,6 +247,7 @@ PROTOTYPES = {
     "grid_wf_propagate": (V, [WF, CG, CX]),
     "grid_wf_propagate_potential": (V, [WF, CX, CG, D]),
     "grid_wf_propagate_kinetic_fft": (V, [WF, CX]),
+    "grid_wf_propagate_kinetic_cfft": (V, [WF, CX]),
     "grid_wf_density": (V, [WF, RG]),
     "grid_wf_zero": (V, [WF]),
     "grid_wf_constant": (V, [WF, CX]),

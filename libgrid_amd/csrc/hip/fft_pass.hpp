@@ -287,6 +287,13 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
     }
+    if (fl & HGRID_PF_KCUT) {
+      // CFFT: components with |k - k0|^2 >= kmax^2 become exact zeros
+      const double kc1 = a.kc_1[c1], kc2 = a.kc_2[c2];
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if ((a.kc_a[t + TPP * j] + kc1) + kc2 >= a.kcut2) v[j] = mk<C>(0, 0);
+    }
   } else if constexpr (KIND == KIND_POT || KIND == KIND_GPE) {
     const Rl mid = (Rl)a.mid;
     const auto rpot = make_rsrc(a.pot, (unsigned long)ntot * CS);

@@ -271,14 +271,17 @@ __global__ void k_potential_phase(C *psi, const C *pot, long n, double cr, doubl
 
 template <typename C>
 __global__ void k_kinetic_phase(C *psi, long nx, long ny, long nz, const double2 *ta, const double2 *tb,
-                                const double2 *tc, const double *scale) {
+                                const double2 *tc, const double *scale, const double *kcx, const double *kcy,
+                                const double *kcz, double kcut2) {
   const long n = nx * ny * nz;
   LG_GRID_STRIDE(idx, n) {
     const long i = idx / (ny * nz), j = (idx / nz) % ny, k = idx % nz;
     // FP64 tables (and pending scale) multiplied in FP64, rounded once on the store
     double2 f = cmul(ta[i], cmul(tb[j], tc[k]));
     if (scale) f = cmul(f, make_double2(scale[0], scale[1]));
-    psi[idx] = cmul_d(psi[idx], f);
+    // CFFT cutoff, |k - k0|^2 summed in the order x, y, z; cut components become exact zeros
+    if (kcx && (kcx[i] + kcy[j]) + kcz[k] >= kcut2) psi[idx] = mk<C>(0, 0);
+    else psi[idx] = cmul_d(psi[idx], f);
   }
 }
 
@@ -465,13 +468,15 @@ extern "C" int hgrid_potential_phase(void *psi, const void *pot, int prec, long 
 }
 
 extern "C" int hgrid_kinetic_phase(void *psi, int prec, long nx, long ny, long nz, const void *ta, const void *tb,
-                                   const void *tc, const double *scale) {
+                                   const void *tc, const double *scale, const double *kcx, const double *kcy,
+                                   const double *kcz, double kcut2) {
   const long n = nx * ny * nz;
+  if (kcx && (!kcy || !kcz)) return -1;
   if (prec == HGRID_C128)
     LG_LAUNCH(k_kinetic_phase<double2>, n, (double2 *)psi, nx, ny, nz, (const double2 *)ta, (const double2 *)tb,
-              (const double2 *)tc, scale);
+              (const double2 *)tc, scale, kcx, kcy, kcz, kcut2);
   LG_LAUNCH(k_kinetic_phase<float2>, n, (float2 *)psi, nx, ny, nz, (const double2 *)ta, (const double2 *)tb,
-            (const double2 *)tc, scale);
+            (const double2 *)tc, scale, kcx, kcy, kcz, kcut2);
 }
 
 extern "C" int hgrid_cscale(void *psi, int prec, long n, double cr, double ci, const double *scale) {

@@ -72,8 +72,10 @@ static void free_ktab(ktab_cache *k) {
   for (int a = 0; a < 3; ++a) {
     if (k->t[a]) hgrid_free(k->t[a]);
     if (k->k2[a]) hgrid_free(k->k2[a]);
+    if (k->kc[a]) hgrid_free(k->kc[a]);
     k->t[a] = NULL;
     k->k2[a] = NULL;
+    k->kc[a] = NULL;
   }
   k->valid = 0;
 }

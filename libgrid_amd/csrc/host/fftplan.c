@@ -136,11 +136,16 @@ ktab_cache *lg_kinetic_tables(cgrid *g, double tm_re, double tm_im, double fold)
       k->kz0 == g->kz0 && k->step == g->step && k->fold == fold && k->nx == g->nx && k->ny == g->ny && k->nz == g->nz)
     return k;
   if (k->valid && (k->nx != g->nx || k->ny != g->ny || k->nz != g->nz || k->prec != d->prec)) {
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
       if (k->t[a]) {
         hgrid_free(k->t[a]);
         k->t[a] = NULL;
       }
+      if (k->kc[a]) {
+        hgrid_free(k->kc[a]);
+        k->kc[a] = NULL;
+      }
+    }
   }
   long n[3] = {g->nx, g->ny, g->nz};
   double k0[3] = {g->kx0, g->ky0, g->kz0};
@@ -177,6 +182,17 @@ ktab_cache *lg_kinetic_tables(cgrid *g, double tm_re, double tm_im, double fold)
       }
     }
     upload_table(&k->t[a], re, im, len);
+    /* (k - k0)^2 as the reference's cutoff test forms it: k in FP64, squared in FP64
+     * (src/grid_wf_fft.c:240-254); a trivial axis holds k0^2 */
+    double *kc = (double *)malloc(sizeof(double) * len);
+    if (!kc) lg_fatal("out of host memory (tables)");
+    for (long i = 0; i < len; ++i) {
+      const double kk = (double)k_of(i, len, g->step, k0[a]);
+      kc[i] = kk * kk;
+    }
+    if (!k->kc[a]) lg_hip(hgrid_malloc((void **)&k->kc[a], sizeof(double) * len), "hipMalloc (cutoff tables)");
+    lg_hip(hgrid_memcpy_h2d(k->kc[a], kc, sizeof(double) * len), "hipMemcpy (cutoff tables)");
+    free(kc);
     free(re);
     free(im);
   }

@@ -114,7 +114,23 @@ void grid_wf_propagate_kinetic_fft(wf *gwf, REAL complex time) {
   const double *scale;
   void *psi = lg_c_inout(g, &scale);
   ktab_cache *k = lg_kinetic_tables(g, creal(tm), cimag(tm), 1.0);
-  lg_hip(hgrid_kinetic_phase(psi, g->dev->prec, g->nx, g->ny, g->nz, k->t[0], k->t[1], k->t[2], scale), "kinetic_phase");
+  lg_hip(hgrid_kinetic_phase(psi, g->dev->prec, g->nx, g->ny, g->nz, k->t[0], k->t[1], k->t[2], scale, NULL, NULL,
+                             NULL, 0.0),
+         "kinetic_phase");
+  lg_c_written(g, 1);
+}
+
+/* reference src/grid_wf_fft.c:206-262: the kinetic phase with components |k - k0| >= kmax
+ * set to zero (the CFFT propagator's high-wavenumber sink) */
+void grid_wf_propagate_kinetic_cfft(wf *gwf, REAL complex time) {
+  cgrid *g = gwf->grid;
+  const REAL complex tm = -I * time * HBAR / (gwf->mass * 2.0);
+  const double *scale;
+  void *psi = lg_c_inout(g, &scale);
+  ktab_cache *k = lg_kinetic_tables(g, creal(tm), cimag(tm), 1.0);
+  lg_hip(hgrid_kinetic_phase(psi, g->dev->prec, g->nx, g->ny, g->nz, k->t[0], k->t[1], k->t[2], scale, k->kc[0],
+                             k->kc[1], k->kc[2], gwf->kmax * gwf->kmax),
+         "kinetic_phase (cutoff)");
   lg_c_written(g, 1);
 }
 
@@ -193,6 +209,16 @@ static void set_k2(hgrid_pass *p, const ktab_cache *k, const lg_shape *s, const 
   p->k2_2 = k2_sq(k, s, g, a2);
 }
 
+static void set_kc(hgrid_pass *p, const ktab_cache *k, const lg_shape *s, const cgrid *g, double kcut2) {
+  int a1, a2;
+  other_axes(p->axis, &a1, &a2);
+  p->flags |= HGRID_PF_KCUT;
+  p->kc_a = k->kc[s->phys[p->axis] >= 0 ? s->phys[p->axis] : trivial_phys(g)];
+  p->kc_1 = k->kc[s->phys[a1] >= 0 ? s->phys[a1] : trivial_phys(g)];
+  p->kc_2 = k->kc[s->phys[a2] >= 0 ? s->phys[a2] : trivial_phys(g)];
+  p->kcut2 = kcut2;
+}
+
 static void *wf_work(cgrid *g) {
   grid_dev *d = g->dev;
   if (!d->work) lg_hip(hgrid_malloc(&d->work, d->n * lg_prec_size(d->prec)), "hipMalloc (wf workspace)");
@@ -208,6 +234,8 @@ typedef struct {
   double g;
   double c_re, c_im, cons;
   double tm_re, tm_im;   /* time_mass of the half step */
+  int kcut;              /* CFFT: zero the components with |k - k0|^2 >= kcut2 */
+  double kcut2;
   int want_energy;       /* reductions for E of the input state */
   int want_post_norm;    /* sum |psi|^2 of the output */
   double renorm_target;  /* > 0: make the output's pending scale renormalise it to this norm */
@@ -297,6 +325,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
     p.flags |= HGRID_PF_MID_KIN;
     if (sa->tm_im == 0.0) p.flags |= HGRID_PF_KIN_REAL; /* exp(tm k^2) with real tm: real tables */
     set_kin(&p, kt, &s, g);
+    if (sa->kcut) set_kc(&p, kt, &s, g, sa->kcut2);
     if (half == 0 && sa->want_energy) {
       p.flags |= HGRID_PF_MID_KRED;
       set_k2(&p, k2, &s, g);
@@ -379,13 +408,23 @@ static void fused_step(wf *gwf, const step_args *sa) {
 /* unfused sequence for grids with fewer than two non-trivial axes (reference order) */
 static void split_step_unfused(wf *gwf, cgrid *potential, REAL complex time, REAL cons) {
   cgrid *g = gwf->grid;
+  const int cut = gwf->propagator == WF_2ND_ORDER_CFFT;
   cgrid_fft(g);
-  grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
+  if (cut) grid_wf_propagate_kinetic_cfft(gwf, 0.5 * time);
+  else grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
   cgrid_inverse_fft_norm(g);
   grid_wf_propagate_potential(gwf, time, potential, cons);
   cgrid_fft(g);
-  grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
+  if (cut) grid_wf_propagate_kinetic_cfft(gwf, 0.5 * time);
+  else grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
   cgrid_inverse_fft_norm(g);
+}
+
+/* the propagators the split-operator kernels implement: plain, and with the k-space cutoff */
+static void check_split_propagator(const wf *gwf, const char *who) {
+  if (gwf->propagator != WF_2ND_ORDER_FFT && gwf->propagator != WF_2ND_ORDER_CFFT)
+    lg_fatal("Error in %s(). Only WF_2ND_ORDER_FFT and WF_2ND_ORDER_CFFT are implemented by this build (propagator %d).",
+             who, (int)gwf->propagator);
 }
 
 static void check_fft_grid(cgrid *g) {
@@ -399,9 +438,7 @@ void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
   cgrid *g = gwf->grid;
   const REAL kx0 = g->kx0, ky0 = g->ky0, kz0 = g->kz0;
   const REAL cons = -(HBAR * HBAR / (2.0 * gwf->mass)) * (kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
-  if (gwf->propagator != WF_2ND_ORDER_FFT)
-    lg_fatal("Error in grid_wf_propagate(). Only WF_2ND_ORDER_FFT is implemented by this build (propagator %d).",
-             (int)gwf->propagator);
+  check_split_propagator(gwf, "grid_wf_propagate");
   if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
   if (gwf->ts_func) lg_fatal("absorbing boundaries (grid_wf_boundary) are not supported by this build");
   check_fft_grid(g);
@@ -425,6 +462,8 @@ void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
   const REAL complex tm = -I * (0.5 * time) * HBAR / (gwf->mass * 2.0);
   sa.tm_re = creal(tm);
   sa.tm_im = cimag(tm);
+  sa.kcut = gwf->propagator == WF_2ND_ORDER_CFFT;
+  sa.kcut2 = gwf->kmax * gwf->kmax;
   fused_step(gwf, &sa);
 }
 
@@ -526,7 +565,7 @@ static double g_last_g = 0.0;
 
 void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize) {
   cgrid *g = gwf->grid;
-  if (gwf->propagator != WF_2ND_ORDER_FFT) lg_fatal("grid_wf_gpe_propagate needs WF_2ND_ORDER_FFT");
+  check_split_propagator(gwf, "grid_wf_gpe_propagate");
   if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
   if (vext->nx != g->nx || vext->ny != g->ny || vext->nz != g->nz) lg_fatal("grid_wf_gpe_propagate: size mismatch");
   check_fft_grid(g);
@@ -545,6 +584,8 @@ void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, c
   const REAL complex tm = -I * (0.5 * time) * HBAR / (gwf->mass * 2.0);
   sa.tm_re = creal(tm);
   sa.tm_im = cimag(tm);
+  sa.kcut = gwf->propagator == WF_2ND_ORDER_CFFT;
+  sa.kcut2 = gwf->kmax * gwf->kmax;
   sa.want_energy = 1;
   sa.want_post_norm = normalize ? 1 : 0;
   sa.renorm_target = normalize ? gwf->norm : 0.0;

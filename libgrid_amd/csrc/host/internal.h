@@ -26,6 +26,7 @@ typedef struct ktab_cache {
   long nx, ny, nz;
   void *t[3];              /* device complex factor tables per physical axis */
   double *k2[3];           /* device |k|^2 tables per physical axis (no k0 shift) */
+  double *kc[3];           /* device (k - k0)^2 tables per physical axis (CFFT cutoff) */
   double host_const_re, host_const_im;  /* product of trivial-axis factors (already folded) */
 } ktab_cache;
 

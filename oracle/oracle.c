@@ -284,6 +284,30 @@ void or_propagate_kinetic_fft(cplx *a, long nx, long ny, long nz, double step, d
   }
 }
 
+/* grid_wf_propagate_kinetic_cfft (src/grid_wf_fft.c:206-262): as the kinetic phase above,
+ * but a component with tot = kx^2 + ky^2 + kz^2 >= kmax^2 is set to zero (k - k0, the same
+ * Nyquist convention, tot summed in the order x, y, z). */
+void or_propagate_kinetic_cfft(cplx *a, long nx, long ny, long nz, double step, double mass, double kx0, double ky0,
+                               double kz0, double time_re, double time_im, double kmax) {
+  long nxy = nx * ny, nx2 = nx / 2, ny2 = ny / 2, nz2 = nz / 2;
+  double lx = 2.0 * M_PI / (step * (double)nx), ly = 2.0 * M_PI / (step * (double)ny), lz = 2.0 * M_PI / (step * (double)nz);
+  double kmax2 = kmax * kmax;
+  cplx time = time_re + time_im * I;
+  cplx time_mass = -I * time * HBAR / (mass * 2.0);
+#pragma omp parallel for schedule(static)
+  for (long ij = 0; ij < nxy; ++ij) {
+    long i = ij / ny, j = ij % ny;
+    double kx = (i <= nx2) ? ((double)i) * lx - kx0 : ((double)(i - nx)) * lx - kx0;
+    double ky = (j <= ny2) ? ((double)j) * ly - ky0 : ((double)(j - ny)) * ly - ky0;
+    for (long k = 0; k < nz; ++k) {
+      double kz = (k <= nz2) ? ((double)k) * lz - kz0 : ((double)(k - nz)) * lz - kz0;
+      double tot = kx * kx + ky * ky + kz * kz;
+      if (tot < kmax2) a[ij * nz + k] *= cexp(time_mass * tot);
+      else a[ij * nz + k] = 0.0;
+    }
+  }
+}
+
 /* grid_wf_propagate_potential (src/grid_wf.c:569-600), FFT propagator (no ts_func):
  * a *= cexp(c * (cons + pot)), c = -i tstep / hbar, pot complex */
 void or_propagate_potential(cplx *a, const cplx *pot, long n, double tre, double tim, double cons) {
@@ -304,6 +328,21 @@ void or_wf_propagate(cplx *a, const cplx *pot, long nx, long ny, long nz, double
   if (pot) or_propagate_potential(a, pot, n, tre, tim, cons);
   or_cgrid_fft(a, nx, ny, nz);
   or_propagate_kinetic_fft(a, nx, ny, nz, step, mass, kx0, ky0, kz0, 0.5 * tre, 0.5 * tim);
+  or_cgrid_inverse_fft_norm(a, nx, ny, nz);
+}
+
+/* grid_wf_propagate, WF_2ND_ORDER_CFFT, no absorbing boundary (src/grid_wf.c:491-498): the
+ * WF_2ND_ORDER_FFT sequence with the cut-off kinetic phase */
+void or_wf_propagate_cfft(cplx *a, const cplx *pot, long nx, long ny, long nz, double step, double mass, double kx0,
+                          double ky0, double kz0, double tre, double tim, double kmax) {
+  double cons = -(HBAR * HBAR / (2.0 * mass)) * (kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
+  long n = nx * ny * nz;
+  or_cgrid_fft(a, nx, ny, nz);
+  or_propagate_kinetic_cfft(a, nx, ny, nz, step, mass, kx0, ky0, kz0, 0.5 * tre, 0.5 * tim, kmax);
+  or_cgrid_inverse_fft_norm(a, nx, ny, nz);
+  if (pot) or_propagate_potential(a, pot, n, tre, tim, cons);
+  or_cgrid_fft(a, nx, ny, nz);
+  or_propagate_kinetic_cfft(a, nx, ny, nz, step, mass, kx0, ky0, kz0, 0.5 * tre, 0.5 * tim, kmax);
   or_cgrid_inverse_fft_norm(a, nx, ny, nz);
 }
 
@@ -361,9 +400,10 @@ double or_kinetic_energy_fd_periodic(const cplx *a, long nx, long ny, long nz, d
 
 /* ---- Gross-Pitaevskii composition of reference calls ---------------------------------------
  * grid_wf_density (src/grid_wf.c:611-627) -> V_eff = vext + g rho -> grid_real_to_complex_re
- * (src/ngrid.c:53-80) -> grid_wf_propagate -> [grid_wf_normalize]. */
+ * (src/ngrid.c:53-80) -> grid_wf_propagate -> [grid_wf_normalize].  kmax < 0 selects the
+ * WF_2ND_ORDER_FFT step, kmax >= 0 the WF_2ND_ORDER_CFFT step with that cutoff. */
 void or_gpe_step(cplx *a, const double *vext, double g, long nx, long ny, long nz, double step, double mass, double kx0,
-                 double ky0, double kz0, double tre, double tim, int normalize, double target) {
+                 double ky0, double kz0, double tre, double tim, int normalize, double target, double kmax) {
   long n = nx * ny * nz;
   cplx *pot = (cplx *)malloc(sizeof(cplx) * (size_t)n);
 #pragma omp parallel for schedule(static)
@@ -371,7 +411,8 @@ void or_gpe_step(cplx *a, const double *vext, double g, long nx, long ny, long n
     double rho = creal(conj(a[i]) * a[i]);
     pot[i] = (cplx)(vext[i] + g * rho);
   }
-  or_wf_propagate(a, pot, nx, ny, nz, step, mass, kx0, ky0, kz0, tre, tim);
+  if (kmax < 0.0) or_wf_propagate(a, pot, nx, ny, nz, step, mass, kx0, ky0, kz0, tre, tim);
+  else or_wf_propagate_cfft(a, pot, nx, ny, nz, step, mass, kx0, ky0, kz0, tre, tim, kmax);
   if (normalize) or_wf_normalize(a, nx, ny, nz, step, target);
   free(pot);
 }

@@ -35,11 +35,13 @@ _PROTOS = {
     "or_propagate_kinetic_fft": (None, [P, L, L, L, D, D, D, D, D, D, D]),
     "or_propagate_potential": (None, [P, P, L, D, D, D]),
     "or_wf_propagate": (None, [P, P, L, L, L, D, D, D, D, D, D, D]),
+    "or_propagate_kinetic_cfft": (None, [P, L, L, L, D, D, D, D, D, D, D, D]),
+    "or_wf_propagate_cfft": (None, [P, P, L, L, L, D, D, D, D, D, D, D, D]),
     "or_wf_normalize": (D, [P, L, L, L, D, D]),
     "or_kinetic_energy_fft": (D, [P, L, L, L, D, D]),
     "or_energy_fft": (D, [P, P, L, L, L, D, D]),
     "or_kinetic_energy_fd_periodic": (D, [P, L, L, L, D, D]),
-    "or_gpe_step": (None, [P, P, D, L, L, L, D, D, D, D, D, D, D, I, D]),
+    "or_gpe_step": (None, [P, P, D, L, L, L, D, D, D, D, D, D, D, I, D, D]),
     "or_gpe_energy": (D, [P, P, D, L, L, L, D, D]),
     "or_num_threads": (I, []),
     "or_set_num_threads": (None, [I]),
@@ -121,6 +123,13 @@ def propagate_kinetic_fft(ak, step, mass, k0, time) -> np.ndarray:
     return b
 
 
+def propagate_kinetic_cfft(ak, step, mass, k0, time, kmax) -> np.ndarray:
+    b = _c(np.array(ak, dtype=np.complex128, order="C", copy=True))
+    t = complex(time)
+    lib().or_propagate_kinetic_cfft(b.ctypes.data, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag, kmax)
+    return b
+
+
 def propagate_potential(a, pot, tstep, cons) -> np.ndarray:
     b = _c(np.array(a, dtype=np.complex128, order="C", copy=True))
     pot = _c(np.ascontiguousarray(pot, dtype=np.complex128))
@@ -147,6 +156,13 @@ def wf_propagate_inplace(b: np.ndarray, pot, step, mass, k0, time) -> None:
     lib().or_wf_propagate(_c(b).ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag)
 
 
+def wf_propagate_cfft_inplace(b: np.ndarray, pot, step, mass, k0, time, kmax) -> None:
+    """One reference grid_wf_propagate (WF_2ND_ORDER_CFFT) step in place; pot may be None."""
+    t = complex(time)
+    pp = None if pot is None else _c(pot).ctypes.data
+    lib().or_wf_propagate_cfft(_c(b).ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag, kmax)
+
+
 def wf_normalize(a, step, target=1.0):
     b = _c(np.array(a, dtype=np.complex128, order="C", copy=True))
     n = lib().or_wf_normalize(b.ctypes.data, *b.shape, step, target)
@@ -169,10 +185,11 @@ def kinetic_energy_fd_periodic(a, step, mass) -> float:
     return lib().or_kinetic_energy_fd_periodic(a.ctypes.data, *a.shape, step, mass)
 
 
-def gpe_step_inplace(b, vext, g, step, mass, k0, time, normalize=True, target=1.0) -> None:
+def gpe_step_inplace(b, vext, g, step, mass, k0, time, normalize=True, target=1.0, kmax=None) -> None:
+    """One GPE step composed of reference calls; kmax selects the CFFT propagator."""
     t = complex(time)
     lib().or_gpe_step(_c(b).ctypes.data, _r(vext).ctypes.data, g, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real,
-                      t.imag, 1 if normalize else 0, target)
+                      t.imag, 1 if normalize else 0, target, -1.0 if kmax is None else kmax)
 
 
 def gpe_energy(a, vext, g, step, mass) -> float:

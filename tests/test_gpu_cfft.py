@@ -1,0 +1,120 @@
+"""GPU parity of the WF_2ND_ORDER_CFFT propagator (reference src/grid_wf.c:491-498 with
+grid_wf_propagate_kinetic_cfft, src/grid_wf_fft.c:206-262): the split step whose kinetic phase
+removes the components with |k - k0| >= kmax, against the CPU oracle on the same inputs.
+
+The cutoff test sums (k - k0)^2 in the reference's order x, y, z, so the set of removed
+components is the oracle's exactly; the bar is the FP64 one of the plain step (relative L2
+<= 1e-12).  kmax is chosen so that a sizeable share of the spectrum is removed.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import libgrid_amd as LG
+from conftest import rel_l2
+from libgrid_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rand_c(shape, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex128)
+
+
+def new_wf(lib, psi, step, mass, kmax):
+    nx, ny, nz = psi.shape
+    w = lib.grid_wf_alloc(nx, ny, nz, step, mass, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_CFFT, b"wf")
+    w.contents.kmax = kmax
+    LG.set_cgrid(w.contents.grid, psi)
+    return w
+
+
+def kmax_for(shape, step, frac):
+    """frac times the largest |k| on the grid (Nyquist along every non-trivial axis)."""
+    kk = sum((np.pi / step) ** 2 for n in shape if n > 1)
+    return frac * np.sqrt(kk)
+
+
+@pytest.mark.parametrize("shape,k0,time", [((16, 32, 64), (0.0, 0.0, 0.0), 0.05), ((8, 8, 8), (0.3, -0.2, 0.1), 0.03 - 0.02j),
+                                           ((1, 64, 32), (0.0, 0.5, 0.0), -0.1j), ((1, 1, 128), (0.0, 0.0, 0.3), 0.02)])
+def test_kinetic_cfft(lg, orc, shape, k0, time):
+    case(f"kinetic cfft {shape} k0={k0} t={time}")
+    step, mass = 0.3, 1.7
+    kmax = kmax_for(shape, step, 0.6)
+    a = rand_c(shape, 5)
+    w = new_wf(lg, a, step, mass, kmax)
+    lg.cgrid_set_momentum(w.contents.grid, *k0)
+    lg.grid_wf_propagate_kinetic_cfft(w, LG.c_complex.of(time))
+    ref = orc.propagate_kinetic_cfft(a, step, mass, k0, time, kmax)
+    out = LG.get_cgrid(w.contents.grid)
+    assert np.array_equal(out == 0, ref == 0), "cut sets differ"
+    assert 0.2 < np.mean(ref == 0) < 0.9
+    assert rel_l2(out, ref) < TOL
+    lg.grid_wf_free(w)
+
+
+@pytest.mark.parametrize("shape,k0,time,frac,nullpot", [
+    ((32, 32, 32), (0, 0, 0), 0.05, 0.5, False),
+    ((16, 32, 64), (0.0, 0.3, -0.2), 0.04 - 0.01j, 0.7, False),
+    ((32, 32, 32), (0.2, 0.0, 0.1), -0.05j, 0.4, False),
+    ((1, 64, 128), (0, 0.2, 0), -0.05j, 0.6, False),
+    ((64, 1, 32), (0, 0, 0), 0.05, 0.6, False),
+    ((1, 1, 256), (0, 0, 0.2), 0.05, 0.5, False),
+    ((32, 16, 16), (0, 0, 0), 0.05, 0.5, True),
+])
+def test_cfft_propagate_matches_oracle(lg, orc, shape, k0, time, frac, nullpot):
+    case(f"cfft propagate {shape} k0={k0} t={time} kmax={frac}*kmax")
+    step, mass = 0.2, 1.0
+    kmax = kmax_for(shape, step, frac)
+    rng = np.random.default_rng(4)
+    psi = W.normalize(rand_c(shape, 12) * 0.1 + 1.0, step)
+    pot = (0.5 * rng.uniform(0, 1, shape) + 0.01j * rng.uniform(0, 1, shape)).astype(np.complex128)
+    w = new_wf(lg, psi, step, mass, kmax)
+    lg.cgrid_set_momentum(w.contents.grid, *k0)
+    vp = None
+    if not nullpot:
+        vp = lg.cgrid_alloc(*shape, step, LG.boundary("cgrid_value_outside_periodic"), None, b"pot")
+        LG.set_cgrid(vp, pot)
+    ref = psi.copy()
+    for _ in range(3):
+        lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
+        orc.wf_propagate_cfft_inplace(ref, None if nullpot else pot, step, mass, k0, time, kmax)
+    assert rel_l2(LG.get_cgrid(w.contents.grid), ref) < TOL
+    # the cutoff did remove something: the plain step differs
+    plain = psi.copy()
+    for _ in range(3):
+        orc.wf_propagate_inplace(plain, None if nullpot else pot, step, mass, k0, time)
+    assert rel_l2(plain, ref) > 1e-6
+    lg.grid_wf_free(w)
+    if vp:
+        lg.cgrid_free(vp)
+
+
+def test_cfft_gpe_step_matches_composition(lg, orc):
+    """The fused Gross-Pitaevskii step with the CFFT propagator (imaginary time, renormalised)
+    against the oracle's composition of reference calls with the cut-off kinetic phase."""
+    case("gpe cfft step")
+    psi, vext, prm = W.gpe_barrier(seed=3, n=32, step=0.8)
+    step, g, dt = prm["step"], prm["g"], prm["dt"]
+    kmax = kmax_for(psi.shape, step, 0.5)
+    w = new_wf(lg, psi, step, 1.0, kmax)
+    vr = lg.rgrid_alloc(*psi.shape, step, LG.boundary("rgrid_value_outside_periodic"), None, b"v")
+    LG.set_rgrid(vr, vext)
+    ref = psi.copy()
+    for _ in range(4):
+        lg.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(-1j * dt), 1)
+        orc.gpe_step_inplace(ref, vext, g, step, 1.0, (0, 0, 0), -1j * dt, normalize=True, kmax=kmax)
+    assert rel_l2(LG.get_cgrid(w.contents.grid), ref) < TOL
+    assert abs(lg.grid_wf_gpe_energy(w, vr, g) - orc.gpe_energy(ref, vext, g, step, 1.0)) < 1e-12 * abs(
+        orc.gpe_energy(ref, vext, g, step, 1.0))
+    lg.grid_wf_free(w)
+    lg.rgrid_free(vr)

@@ -6,7 +6,9 @@ so each oracle function is checked against something it does not contain:
     DFTs, Parseval / round trip / shift theorem / plane waves, and the cosine round trip of
     the reference's examples/cfft.c;
   * kinetic phase: closed-form free-particle evolution of a Gaussian packet (also in a moving
-    frame, k0 != 0) and the reference's Nyquist convention;
+    frame, k0 != 0) and the reference's Nyquist convention; the cut-off (CFFT) variant against
+    its limits (no cut, everything cut), the Nyquist convention of its test and plane-wave
+    eigenstates inside and outside the cutoff sphere;
   * potential phase and the full step: constant potential = global phase, absorbing
     potential = decay, Ehrenfest dynamics in a harmonic trap, time reversibility, norm
     conservation, the harmonic-oscillator ground state E0 = 3/2 hbar omega;
@@ -148,6 +150,52 @@ def test_kinetic_nyquist_convention(orc):
     out = orc.propagate_kinetic_fft(ak, step, mass, k0, 0.1)
     kn = (n // 2) * 2 * np.pi / (n * step) - k0[2]
     assert abs(out[0, 0, n // 2] - np.exp(-1j * 0.1 * kn * kn / (2 * mass))) < 1e-14
+
+
+def test_cfft_cutoff_limits_and_nyquist(orc):
+    """The cut-off kinetic phase (src/grid_wf_fft.c:206-262): with a cutoff above every |k - k0|
+    it is the plain kinetic phase, with kmax = 0 it zeroes everything, and the Nyquist index
+    is +n/2 in the test tot < kmax^2 as well."""
+    rng = np.random.default_rng(21)
+    shape, step, mass, k0, t = (8, 12, 16), 0.25, 1.3, (0.1, -0.2, 0.4), 0.07 - 0.02j
+    ak = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    plain = orc.propagate_kinetic_fft(ak, step, mass, k0, t)
+    np.testing.assert_array_equal(orc.propagate_kinetic_cfft(ak, step, mass, k0, t, 1e4), plain)
+    assert not np.any(orc.propagate_kinetic_cfft(ak, step, mass, k0, t, 0.0))
+    n = 16
+    one = np.zeros((1, 1, n), complex)
+    one[0, 0, n // 2] = 1.0
+    l = 2 * np.pi / (n * step)
+    kn_pos, kn_neg = (n // 2) * l - k0[2], -(n // 2) * l - k0[2]
+    kmax = np.sqrt(0.5 * ((kn_pos**2 + k0[0] ** 2 + k0[1] ** 2) + (kn_neg**2 + k0[0] ** 2 + k0[1] ** 2)))
+    out = orc.propagate_kinetic_cfft(one, step, mass, k0, 0.1, kmax)
+    tot = k0[0] ** 2 + k0[1] ** 2 + kn_pos**2
+    assert abs(out[0, 0, n // 2] - np.exp(-1j * 0.1 * tot / (2 * mass))) < 1e-14
+
+
+def test_cfft_step_plane_waves_closed_form(orc):
+    """Free plane waves are eigenstates of the split step: the CFFT step keeps each wave with
+    |k| < kmax, multiplied by exp(-i hbar k^2 t / 2m), and removes the others.  The set holds a
+    wave whose components are each below kmax while |k| is above it (the test is on |k|^2, not
+    per axis), and waves just inside and just outside the sphere."""
+    shape, step, mass, t = (16, 16, 16), 0.4, 1.0, 0.3
+    x, y, z = W.coords(shape, step)
+    l = 2 * np.pi / (16 * step)
+    modes = [((1, 0, 0), 1.0), ((2, -3, 1), 0.5 - 0.2j), ((-3, 3, 3), 0.3j), ((0, 4, 0), 0.7), ((-4, 0, 1), -0.4)]
+    kmax = np.sqrt(16.5) * l  # |(-4,0,1)| = sqrt(17) l is cut, |(0,4,0)| = 4 l is kept
+    psi = np.zeros(shape, complex)
+    exact = np.zeros(shape, complex)
+    for m, c in modes:
+        k = np.array(m) * l
+        wave = c * np.exp(1j * (k[0] * x + k[1] * y + k[2] * z))
+        psi += wave
+        if k @ k < kmax**2:
+            exact += wave * np.exp(-1j * t * (k @ k) / (2 * mass))
+    kept = [m for m, _ in modes if np.dot(m, m) * l * l < kmax**2]
+    assert kept == [(1, 0, 0), (2, -3, 1), (0, 4, 0)]
+    psi = np.ascontiguousarray(psi)
+    orc.wf_propagate_cfft_inplace(psi, None, step, mass, (0.0, 0.0, 0.0), t, kmax)
+    assert rel_l2(psi, exact) < 1e-13
 
 
 def test_constant_and_absorbing_potential(orc):
