@@ -59,10 +59,10 @@
 #define WF_NEUMANN_BOUNDARY 2
 #define WF_PERIODIC_BOUNDARY 3
 
-/* Wavefunction propagators (reference src/defs.h:26-32).  This build implements
- * WF_2ND_ORDER_FFT and WF_2ND_ORDER_CFFT (the same step with components |k - k0| >= kmax
- * removed in the kinetic phase); grid_wf_alloc accepts the others and grid_wf_propagate aborts
- * for them. */
+/* Wavefunction propagators (reference src/defs.h:26-32).  This build implements the FFT ones
+ * (1-4): WF_2ND_ORDER_FFT, WF_4TH_ORDER_FFT, and their CFFT forms, which remove the components
+ * |k - k0| >= kmax in the kinetic phase.  grid_wf_alloc accepts the others and
+ * grid_wf_propagate aborts for them. */
 #define WF_2ND_ORDER_FFT 1
 #define WF_4TH_ORDER_FFT 2
 #define WF_2ND_ORDER_CFFT 3

@@ -234,6 +234,7 @@ typedef struct {
   double g;
   double c_re, c_im, cons;
   double tm_re, tm_im;   /* time_mass of the half step */
+  int no_defer;          /* leave the result in real space (no z transform kept pending) */
   int kcut;              /* CFFT: zero the components with |k - k0|^2 >= kcut2 */
   double kcut2;
   int want_energy;       /* reductions for E of the input state */
@@ -255,7 +256,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
    * FP64 arithmetic: with FP32 butterflies, imaginary-time relaxation from a transformed state
    * needed about four times the steps to meet the stopping rule, so FP32 grids keep the
    * real-space state between steps. */
-  const int defer = s.m == 3 && prec != HGRID_C64;
+  const int defer = s.m == 3 && prec != HGRID_C64 && !sa->no_defer;
   if (d->zpend && !defer) lg_c_materialize(g);
   const int zin = d->zpend;
   ktab_cache *kt = lg_kinetic_tables(g, sa->tm_re, sa->tm_im, g->fft_norm);
@@ -405,26 +406,44 @@ static void fused_step(wf *gwf, const step_args *sa) {
   d->zpend = defer;
 }
 
-/* unfused sequence for grids with fewer than two non-trivial axes (reference order) */
-static void split_step_unfused(wf *gwf, cgrid *potential, REAL complex time, REAL cons) {
-  cgrid *g = gwf->grid;
-  const int cut = gwf->propagator == WF_2ND_ORDER_CFFT;
-  cgrid_fft(g);
-  if (cut) grid_wf_propagate_kinetic_cfft(gwf, 0.5 * time);
-  else grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
-  cgrid_inverse_fft_norm(g);
-  grid_wf_propagate_potential(gwf, time, potential, cons);
-  cgrid_fft(g);
-  if (cut) grid_wf_propagate_kinetic_cfft(gwf, 0.5 * time);
-  else grid_wf_propagate_kinetic_fft(gwf, 0.5 * time);
-  cgrid_inverse_fft_norm(g);
+/* the propagator branches of grid_wf_propagate (reference src/grid_wf.c:390-525) */
+static int uses_cutoff(const wf *gwf) {
+  return gwf->propagator == WF_2ND_ORDER_CFFT || gwf->propagator == WF_4TH_ORDER_CFFT;
+}
+static int fourth_order(const wf *gwf) {
+  return gwf->propagator == WF_4TH_ORDER_FFT || gwf->propagator == WF_4TH_ORDER_CFFT;
 }
 
-/* the propagators the split-operator kernels implement: plain, and with the k-space cutoff */
-static void check_split_propagator(const wf *gwf, const char *who) {
-  if (gwf->propagator != WF_2ND_ORDER_FFT && gwf->propagator != WF_2ND_ORDER_CFFT)
-    lg_fatal("Error in %s(). Only WF_2ND_ORDER_FFT and WF_2ND_ORDER_CFFT are implemented by this build (propagator %d).",
-             who, (int)gwf->propagator);
+static void kinetic_unfused(wf *gwf, REAL complex time) {
+  cgrid_fft(gwf->grid);
+  if (uses_cutoff(gwf)) grid_wf_propagate_kinetic_cfft(gwf, time);
+  else grid_wf_propagate_kinetic_fft(gwf, time);
+  cgrid_inverse_fft_norm(gwf->grid);
+}
+
+/* unfused sequence for grids with fewer than two non-trivial axes (reference order):
+ * K(t/2) V(t) K(t/2), or V(t/6) K(t/2) V(2t/3) K(t/2) V(t/6) for the 4th-order branches */
+static void split_step_unfused(wf *gwf, cgrid *potential, REAL complex time, REAL cons) {
+  if (fourth_order(gwf)) {
+    grid_wf_propagate_potential(gwf, time / 6.0, potential, cons);
+    kinetic_unfused(gwf, 0.5 * time);
+    grid_wf_propagate_potential(gwf, 2.0 * time / 3.0, potential, cons);
+    kinetic_unfused(gwf, 0.5 * time);
+    grid_wf_propagate_potential(gwf, time / 6.0, potential, cons);
+    return;
+  }
+  kinetic_unfused(gwf, 0.5 * time);
+  grid_wf_propagate_potential(gwf, time, potential, cons);
+  kinetic_unfused(gwf, 0.5 * time);
+}
+
+/* the propagators the split-operator kernels implement */
+static void check_split_propagator(const wf *gwf, const char *who, int allow_fourth) {
+  const char p = gwf->propagator;
+  if (p == WF_2ND_ORDER_FFT || p == WF_2ND_ORDER_CFFT) return;
+  if (allow_fourth && (p == WF_4TH_ORDER_FFT || p == WF_4TH_ORDER_CFFT)) return;
+  lg_fatal("Error in %s(). Propagator %d is not implemented by this build (%s).", who, (int)p,
+           allow_fourth ? "FFT propagators 1-4 are" : "WF_2ND_ORDER_FFT and WF_2ND_ORDER_CFFT are");
 }
 
 static void check_fft_grid(cgrid *g) {
@@ -438,7 +457,7 @@ void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
   cgrid *g = gwf->grid;
   const REAL kx0 = g->kx0, ky0 = g->ky0, kz0 = g->kz0;
   const REAL cons = -(HBAR * HBAR / (2.0 * gwf->mass)) * (kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
-  check_split_propagator(gwf, "grid_wf_propagate");
+  check_split_propagator(gwf, "grid_wf_propagate", 1);
   if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
   if (gwf->ts_func) lg_fatal("absorbing boundaries (grid_wf_boundary) are not supported by this build");
   check_fft_grid(g);
@@ -456,15 +475,24 @@ void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
     if (potential->dev->prec != g->dev->prec) cgrid_set_precision(potential, cgrid_get_precision(g));
     sa.pot = lg_c_in_flushed(potential);
   }
-  sa.c_re = cimag(time) / HBAR;
-  sa.c_im = -creal(time) / HBAR;
+  /* 4th order (reference src/grid_wf.c:433-457): the outer V(t/6) stages run as pointwise
+   * sweeps around a fused K(t/2) V(2t/3) K(t/2).  The reference also forms
+   * (1/48)(hbar^2/m)|t|^2 |grad V|^2 in a workspace (:449-451) but hands the unmodified
+   * potential to the V(2t/3) stage (:452); that term never reaches psi, so it is not formed. */
+  const int fourth = fourth_order(gwf);
+  const REAL complex tv = fourth ? 2.0 * time / 3.0 : time;
+  if (fourth) grid_wf_propagate_potential(gwf, time / 6.0, potential, cons);
+  sa.c_re = cimag(tv) / HBAR;
+  sa.c_im = -creal(tv) / HBAR;
   sa.cons = cons;
   const REAL complex tm = -I * (0.5 * time) * HBAR / (gwf->mass * 2.0);
   sa.tm_re = creal(tm);
   sa.tm_im = cimag(tm);
-  sa.kcut = gwf->propagator == WF_2ND_ORDER_CFFT;
+  sa.kcut = uses_cutoff(gwf);
   sa.kcut2 = gwf->kmax * gwf->kmax;
+  sa.no_defer = fourth; /* the last V(t/6) stage reads the real-space state */
   fused_step(gwf, &sa);
+  if (fourth) grid_wf_propagate_potential(gwf, time / 6.0, potential, cons);
 }
 
 /* ---- energies ----------------------------------------------------------------------------- */
@@ -565,7 +593,7 @@ static double g_last_g = 0.0;
 
 void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize) {
   cgrid *g = gwf->grid;
-  check_split_propagator(gwf, "grid_wf_gpe_propagate");
+  check_split_propagator(gwf, "grid_wf_gpe_propagate", 0);
   if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
   if (vext->nx != g->nx || vext->ny != g->ny || vext->nz != g->nz) lg_fatal("grid_wf_gpe_propagate: size mismatch");
   check_fft_grid(g);
@@ -584,7 +612,7 @@ void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, c
   const REAL complex tm = -I * (0.5 * time) * HBAR / (gwf->mass * 2.0);
   sa.tm_re = creal(tm);
   sa.tm_im = cimag(tm);
-  sa.kcut = gwf->propagator == WF_2ND_ORDER_CFFT;
+  sa.kcut = uses_cutoff(gwf);
   sa.kcut2 = gwf->kmax * gwf->kmax;
   sa.want_energy = 1;
   sa.want_post_norm = normalize ? 1 : 0;

@@ -346,6 +346,27 @@ void or_wf_propagate_cfft(cplx *a, const cplx *pot, long nx, long ny, long nz, d
   or_cgrid_inverse_fft_norm(a, nx, ny, nz);
 }
 
+/* grid_wf_propagate, WF_4TH_ORDER_FFT / WF_4TH_ORDER_CFFT without an absorbing boundary
+ * (src/grid_wf.c:433-457, :505-525): V(t/6), K(t/2), V(2t/3), K(t/2), V(t/6), each V with the
+ * same cons as the 2nd-order step.  The reference also forms (1/48)(hbar^2/m)|t|^2 |grad V|^2
+ * in a workspace (:449-451) but passes the unmodified potential to the V(2t/3) stage (:452),
+ * so that term never reaches psi and is not formed here.  kmax < 0: plain kinetic phase;
+ * kmax >= 0: the cut-off one. */
+void or_wf_propagate_4th(cplx *a, const cplx *pot, long nx, long ny, long nz, double step, double mass, double kx0,
+                         double ky0, double kz0, double tre, double tim, double kmax) {
+  double cons = -(HBAR * HBAR / (2.0 * mass)) * (kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
+  long n = nx * ny * nz;
+  cplx t = tre + tim * I, t6 = t / 6.0, t23 = 2.0 * t / 3.0, th = 0.5 * t;
+  for (int half = 0; half < 2; ++half) {
+    if (pot) or_propagate_potential(a, pot, n, creal(half ? t23 : t6), cimag(half ? t23 : t6), cons);
+    or_cgrid_fft(a, nx, ny, nz);
+    if (kmax < 0.0) or_propagate_kinetic_fft(a, nx, ny, nz, step, mass, kx0, ky0, kz0, creal(th), cimag(th));
+    else or_propagate_kinetic_cfft(a, nx, ny, nz, step, mass, kx0, ky0, kz0, creal(th), cimag(th), kmax);
+    or_cgrid_inverse_fft_norm(a, nx, ny, nz);
+  }
+  if (pot) or_propagate_potential(a, pot, n, creal(t6), cimag(t6), cons);
+}
+
 /* grid_wf_normalize (src/grid_wf.c:694-700): multiply by sqrt(target / norm); returns norm */
 double or_wf_normalize(cplx *a, long nx, long ny, long nz, double step, double target) {
   double norm = or_integral_of_square(a, nx, ny, nz, step);

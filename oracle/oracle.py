@@ -37,6 +37,7 @@ _PROTOS = {
     "or_wf_propagate": (None, [P, P, L, L, L, D, D, D, D, D, D, D]),
     "or_propagate_kinetic_cfft": (None, [P, L, L, L, D, D, D, D, D, D, D, D]),
     "or_wf_propagate_cfft": (None, [P, P, L, L, L, D, D, D, D, D, D, D, D]),
+    "or_wf_propagate_4th": (None, [P, P, L, L, L, D, D, D, D, D, D, D, D]),
     "or_wf_normalize": (D, [P, L, L, L, D, D]),
     "or_kinetic_energy_fft": (D, [P, L, L, L, D, D]),
     "or_energy_fft": (D, [P, P, L, L, L, D, D]),
@@ -161,6 +162,15 @@ def wf_propagate_cfft_inplace(b: np.ndarray, pot, step, mass, k0, time, kmax) ->
     t = complex(time)
     pp = None if pot is None else _c(pot).ctypes.data
     lib().or_wf_propagate_cfft(_c(b).ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag, kmax)
+
+
+def wf_propagate_4th_inplace(b: np.ndarray, pot, step, mass, k0, time, kmax=None) -> None:
+    """One reference grid_wf_propagate WF_4TH_ORDER_FFT step in place (WF_4TH_ORDER_CFFT when
+    kmax is given); pot may be None."""
+    t = complex(time)
+    pp = None if pot is None else _c(pot).ctypes.data
+    lib().or_wf_propagate_4th(_c(b).ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag,
+                              -1.0 if kmax is None else kmax)
 
 
 def wf_normalize(a, step, target=1.0):

@@ -1,10 +1,14 @@
-"""GPU parity of the WF_2ND_ORDER_CFFT propagator (reference src/grid_wf.c:491-498 with
-grid_wf_propagate_kinetic_cfft, src/grid_wf_fft.c:206-262): the split step whose kinetic phase
-removes the components with |k - k0| >= kmax, against the CPU oracle on the same inputs.
+"""GPU parity of the propagator variants against the CPU oracle on the same inputs:
 
-The cutoff test sums (k - k0)^2 in the reference's order x, y, z, so the set of removed
-components is the oracle's exactly; the bar is the FP64 one of the plain step (relative L2
-<= 1e-12).  kmax is chosen so that a sizeable share of the spectrum is removed.
+  * WF_2ND_ORDER_CFFT (reference src/grid_wf.c:491-498 with grid_wf_propagate_kinetic_cfft,
+    src/grid_wf_fft.c:206-262): the split step whose kinetic phase removes the components
+    with |k - k0| >= kmax.  The cutoff test sums (k - k0)^2 in the reference's order x, y, z,
+    so the set of removed components is the oracle's exactly.  kmax is chosen so that a
+    sizeable share of the spectrum is removed.
+  * WF_4TH_ORDER_FFT / WF_4TH_ORDER_CFFT (src/grid_wf.c:433-457, :505-525):
+    V(t/6) K(t/2) V(2t/3) K(t/2) V(t/6).
+
+The bar is the FP64 one of the plain step (relative L2 <= 1e-12).
 """
 import sys
 
@@ -30,10 +34,11 @@ def rand_c(shape, seed):
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex128)
 
 
-def new_wf(lib, psi, step, mass, kmax):
+def new_wf(lib, psi, step, mass, kmax, propagator=LG.WF_2ND_ORDER_CFFT):
     nx, ny, nz = psi.shape
-    w = lib.grid_wf_alloc(nx, ny, nz, step, mass, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_CFFT, b"wf")
-    w.contents.kmax = kmax
+    w = lib.grid_wf_alloc(nx, ny, nz, step, mass, LG.WF_PERIODIC_BOUNDARY, propagator, b"wf")
+    if kmax is not None:
+        w.contents.kmax = kmax
     LG.set_cgrid(w.contents.grid, psi)
     return w
 
@@ -118,3 +123,61 @@ def test_cfft_gpe_step_matches_composition(lg, orc):
         orc.gpe_energy(ref, vext, g, step, 1.0))
     lg.grid_wf_free(w)
     lg.rgrid_free(vr)
+
+
+@pytest.mark.parametrize("shape,k0,time,frac,nullpot", [
+    ((32, 32, 32), (0, 0, 0), 0.05, None, False),
+    ((16, 32, 64), (0.0, 0.3, -0.2), 0.04 - 0.01j, None, False),
+    ((32, 32, 32), (0, 0, 0), -0.05j, None, False),
+    ((1, 64, 128), (0, 0.2, 0), 0.05, None, False),
+    ((1, 1, 256), (0, 0, 0.2), 0.05, None, False),
+    ((32, 16, 16), (0, 0, 0), 0.05, None, True),
+    ((32, 32, 32), (0.2, 0.0, 0.1), 0.05, 0.5, False),
+    ((1, 1, 256), (0, 0, 0), -0.05j, 0.5, False),
+])
+def test_fourth_order_matches_oracle(lg, orc, shape, k0, time, frac, nullpot):
+    case(f"4th order {shape} k0={k0} t={time} cutoff={frac} nullpot={nullpot}")
+    step, mass = 0.2, 1.0
+    kmax = None if frac is None else kmax_for(shape, step, frac)
+    prop = LG.WF_4TH_ORDER_FFT if frac is None else LG.WF_4TH_ORDER_CFFT
+    rng = np.random.default_rng(4)
+    psi = W.normalize(rand_c(shape, 12) * 0.1 + 1.0, step)
+    pot = (0.5 * rng.uniform(0, 1, shape) + 0.01j * rng.uniform(0, 1, shape)).astype(np.complex128)
+    w = new_wf(lg, psi, step, mass, kmax, prop)
+    lg.cgrid_set_momentum(w.contents.grid, *k0)
+    vp = None
+    if not nullpot:
+        vp = lg.cgrid_alloc(*shape, step, LG.boundary("cgrid_value_outside_periodic"), None, b"pot")
+        LG.set_cgrid(vp, pot)
+    ref = psi.copy()
+    for _ in range(3):
+        lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
+        orc.wf_propagate_4th_inplace(ref, None if nullpot else pot, step, mass, k0, time, kmax)
+    assert rel_l2(LG.get_cgrid(w.contents.grid), ref) < TOL
+    lg.grid_wf_free(w)
+    if vp:
+        lg.cgrid_free(vp)
+
+
+def test_fourth_order_after_second_order_steps(lg, orc):
+    """A 3-D FP64 state left transformed along z by 2nd-order steps is brought back to real
+    space before a 4th-order step (the propagator field switched between calls)."""
+    case("2nd then 4th order, 32^3")
+    shape, step, mass, time = (32, 32, 32), 0.2, 1.0, 0.05
+    rng = np.random.default_rng(5)
+    psi = W.normalize(rand_c(shape, 13) * 0.1 + 1.0, step)
+    pot = (0.5 * rng.uniform(0, 1, shape) + 0j).astype(np.complex128)
+    w = new_wf(lg, psi, step, mass, None, LG.WF_2ND_ORDER_FFT)
+    vp = lg.cgrid_alloc(*shape, step, LG.boundary("cgrid_value_outside_periodic"), None, b"pot")
+    LG.set_cgrid(vp, pot)
+    ref = psi.copy()
+    for _ in range(2):
+        lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
+        orc.wf_propagate_inplace(ref, pot, step, mass, (0, 0, 0), time)
+    w.contents.propagator = LG.WF_4TH_ORDER_FFT
+    for _ in range(2):
+        lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
+        orc.wf_propagate_4th_inplace(ref, pot, step, mass, (0, 0, 0), time)
+    assert rel_l2(LG.get_cgrid(w.contents.grid), ref) < TOL
+    lg.grid_wf_free(w)
+    lg.cgrid_free(vp)

@@ -198,6 +198,71 @@ def test_cfft_step_plane_waves_closed_form(orc):
     assert rel_l2(psi, exact) < 1e-13
 
 
+def gaussian_fixed_point(a, b, dt, mass, kappa):
+    """Width of the Gaussian exp(-alpha r^2 / 2) that one imaginary-time step
+    V(a dt) K(dt/2) V(b dt) K(dt/2) V(a dt) maps onto itself in V = kappa r^2 / 2 (hbar = 1):
+    V(s) adds kappa s to alpha and K(s) maps 1/alpha to 1/alpha + s/mass, exactly, because
+    both act on a Gaussian in closed form.  Iterated to convergence in plain floats."""
+    al = 1.0
+    for _ in range(2000):
+        al = 1.0 / (1.0 / (al + kappa * a * dt) + 0.5 * dt / mass)
+        al = 1.0 / (1.0 / (al + kappa * b * dt) + 0.5 * dt / mass) + kappa * a * dt
+    return al
+
+
+@pytest.mark.parametrize("order", ["2nd", "4th"])
+def test_splitting_fixed_point_closed_form(orc, order):
+    """Imaginary-time steps in a harmonic trap converge to the splitting's own Gaussian (not to
+    the exact ground state: the difference is O(dt^2) and depends on the stage coefficients).
+    2nd order is K(t/2) V(t) K(t/2) (a = 0, b = 1); the reference's 4th-order branch is
+    V(t/6) K(t/2) V(2t/3) K(t/2) V(t/6) without the |grad V|^2 term, which it forms but does
+    not use (src/grid_wf.c:449-452).  At dt = 0.5 the widths of a = 1/6, 1/4 and 0 differ by
+    about 1 %, so a wrong stage coefficient fails here."""
+    n, step, mass, kappa, dt = 48, 0.35, 1.3, 1.0, 0.5
+    a, b = (0.0, 1.0) if order == "2nd" else (1.0 / 6.0, 2.0 / 3.0)
+    shape = (n, n, n)
+    x, y, z = W.coords(shape, step)
+    r2 = np.broadcast_to(x * x + y * y + z * z, shape)
+    pot = np.ascontiguousarray(0.5 * kappa * r2 + 0j)
+    psi = np.ascontiguousarray(np.exp(-0.5 * r2) + 0j)
+    for _ in range(60):
+        if order == "2nd":
+            orc.wf_propagate_inplace(psi, pot, step, mass, (0, 0, 0), -1j * dt)
+        else:
+            orc.wf_propagate_4th_inplace(psi, pot, step, mass, (0, 0, 0), -1j * dt)
+        psi, _ = orc.wf_normalize(psi, step)
+    al = gaussian_fixed_point(a, b, dt, mass, kappa)
+    exact = np.exp(-0.5 * al * r2)
+    exact *= np.sqrt(1.0 / (np.sum(exact**2) * step**3))
+    assert rel_l2(psi, exact) < 1e-10
+    e = orc.energy_fft(psi, 0.5 * kappa * np.ascontiguousarray(r2), step, mass)
+    assert abs(e - 3.0 * (al / (4.0 * mass) + kappa / (4.0 * al))) < 1e-10
+
+
+def test_4th_order_limits_and_reversibility(orc):
+    """Without a potential the 4th-order step is the 2nd-order one (the same FFTs and kinetic
+    phases in the same order, bit for bit); a constant potential V0 is the global phase
+    exp(-i (V0 + cons) t) on top of free motion; the symmetric stage sequence is reversible."""
+    rng = np.random.default_rng(8)
+    shape, step, mass = (16, 12, 8), 0.3, 1.1
+    psi = np.ascontiguousarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    a4, a2 = psi.copy(), psi.copy()
+    orc.wf_propagate_4th_inplace(a4, None, step, mass, (0.2, 0, -0.1), 0.05)
+    orc.wf_propagate_inplace(a2, None, step, mass, (0.2, 0, -0.1), 0.05)
+    np.testing.assert_array_equal(a4, a2)
+    v0 = 0.7 - 0.05j
+    a4, a2 = psi.copy(), psi.copy()
+    orc.wf_propagate_4th_inplace(a4, np.full(shape, v0), step, mass, (0, 0, 0), 0.05)
+    orc.wf_propagate_inplace(a2, None, step, mass, (0, 0, 0), 0.05)
+    assert rel_l2(a4, a2 * np.exp(-1j * v0 * 0.05)) < 1e-13
+    v = np.ascontiguousarray(rng.uniform(0, 2, shape) + 0j)
+    b = psi.copy()
+    orc.wf_propagate_4th_inplace(b, v, step, mass, (0, 0, 0), 0.07)
+    assert rel_l2(b, psi) > 1e-3
+    orc.wf_propagate_4th_inplace(b, v, step, mass, (0, 0, 0), -0.07)
+    assert rel_l2(b, psi) < 1e-13
+
+
 def test_constant_and_absorbing_potential(orc):
     """A constant potential V0 is a global phase exp(-i V0 t); V0 - i G also decays as exp(-G t)."""
     rng = np.random.default_rng(5)
