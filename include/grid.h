@@ -226,6 +226,12 @@ void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time);          /
 void grid_wf_propagate_potential(wf *gwf, REAL complex tstep, cgrid *potential, REAL cons); /* :569 */
 void grid_wf_propagate_kinetic_fft(wf *gwf, REAL complex time);                /* grid_wf_fft.c:141 */
 void grid_wf_propagate_kinetic_cfft(wf *gwf, REAL complex time);               /* grid_wf_fft.c:206 */
+/* Absorbing boundary for the 2nd-order FFT propagators: outside [lx,hx] x [ly,hy] x [lz,hz] a
+ * real-time step is blended with an imaginary-time one by grid_wf_absorb's amplitude.  All
+ * bounds 0 clears it. */
+void grid_wf_boundary(wf *gwf, wf *gwfp, INT lx, INT hx, INT ly, INT hy, INT lz, INT hz); /* :99 */
+REAL grid_wf_absorb(INT i, INT j, INT k, INT lx, INT hx, INT ly, INT hy, INT lz, INT hz); /* :185 */
+void grid_wf_merge(wf *dst, cgrid *wfr, cgrid *wfi);                           /* :728 */
 void grid_wf_density(wf *gwf, rgrid *density);                                  /* :611 */
 void grid_wf_zero(wf *gwf);
 void grid_wf_constant(wf *gwf, REAL complex c);

@@ -248,6 +248,9 @@ PROTOTYPES = {
     "grid_wf_propagate_potential": (V, [WF, CX, CG, D]),
     "grid_wf_propagate_kinetic_fft": (V, [WF, CX]),
     "grid_wf_propagate_kinetic_cfft": (V, [WF, CX]),
+    "grid_wf_boundary": (V, [WF, WF, L, L, L, L, L, L]),
+    "grid_wf_absorb": (D, [L, L, L, L, L, L, L, L, L]),
+    "grid_wf_merge": (V, [WF, CG, CG]),
     "grid_wf_density": (V, [WF, RG]),
     "grid_wf_zero": (V, [WF]),
     "grid_wf_constant": (V, [WF, CX]),

@@ -285,6 +285,35 @@ __global__ void k_kinetic_phase(C *psi, long nx, long ny, long nz, const double2
   }
 }
 
+// absorbing-boundary amplitude: 0 inside [lx, hx] x [ly, hy] x [lz, hz]; outside, the distance
+// over the lower bound per axis (both sides), summed, times 2/3, capped at 1
+__device__ static double absorb_amp(long i, long j, long k, long lx, long hx, long ly, long hy, long lz, long hz) {
+  double t = 0.0;
+  if (i < lx) t += (double)(lx - i) / (double)lx;
+  else if (i > hx) t += (double)(i - hx) / (double)lx;
+  if (j < ly) t += (double)(ly - j) / (double)ly;
+  else if (j > hy) t += (double)(j - hy) / (double)ly;
+  if (k < lz) t += (double)(lz - k) / (double)lz;
+  else if (k > hz) t += (double)(k - hz) / (double)lz;
+  t *= 2.0 / 3.0;
+  return t > 1.0 ? 1.0 : t;
+}
+
+// dst = (1 - alpha) wr + alpha wi (dst may be wr); products and sums rounded separately
+template <typename C>
+__global__ void k_wf_merge(C *dst, const C *wr, const C *wi, long nx, long ny, long nz, long lx, long hx, long ly,
+                           long hy, long lz, long hz) {
+#pragma clang fp contract(off)
+  const long n = nx * ny * nz;
+  LG_GRID_STRIDE(idx, n) {
+    const long i = idx / (ny * nz), j = (idx / nz) % ny, k = idx % nz;
+    const double al = absorb_amp(i, j, k, lx, hx, ly, hy, lz, hz), bl = 1.0 - al;
+    const C r = wr[idx], m = wi[idx];
+    const double re = bl * (double)r.x + al * (double)m.x, im = bl * (double)r.y + al * (double)m.y;
+    dst[idx] = mk<C>((typename real_of<C>::type)re, (typename real_of<C>::type)im);
+  }
+}
+
 template <typename C> __global__ void k_cscale(C *psi, long n, double cr, double ci, const double *scale) {
   C f = mk<C>((typename real_of<C>::type)cr, (typename real_of<C>::type)ci);
   if (scale) {
@@ -477,6 +506,16 @@ extern "C" int hgrid_kinetic_phase(void *psi, int prec, long nx, long ny, long n
               (const double2 *)tc, scale, kcx, kcy, kcz, kcut2);
   LG_LAUNCH(k_kinetic_phase<float2>, n, (float2 *)psi, nx, ny, nz, (const double2 *)ta, (const double2 *)tb,
             (const double2 *)tc, scale, kcx, kcy, kcz, kcut2);
+}
+
+extern "C" int hgrid_wf_merge(void *dst, const void *wr, const void *wi, int prec, long nx, long ny, long nz,
+                              const long *bounds) {
+  const long n = nx * ny * nz;
+  if (prec == HGRID_C128)
+    LG_LAUNCH(k_wf_merge<double2>, n, (double2 *)dst, (const double2 *)wr, (const double2 *)wi, nx, ny, nz, bounds[0],
+              bounds[1], bounds[2], bounds[3], bounds[4], bounds[5]);
+  LG_LAUNCH(k_wf_merge<float2>, n, (float2 *)dst, (const float2 *)wr, (const float2 *)wi, nx, ny, nz, bounds[0],
+            bounds[1], bounds[2], bounds[3], bounds[4], bounds[5]);
 }
 
 extern "C" int hgrid_cscale(void *psi, int prec, long n, double cr, double ci, const double *scale) {

@@ -394,6 +394,14 @@ void cgrid_constant(cgrid *grid, REAL complex c) {
 
 void cgrid_copy(cgrid *dst, cgrid *src) {
   if (dst->nx != src->nx || dst->ny != src->ny || dst->nz != src->nz) lg_fatal("Incompatible grids in cgrid_copy().");
+  /* the grid geometry and the moving frame travel with the values (reference src/cgrid.c:626-632) */
+  dst->step = src->step;
+  dst->x0 = src->x0;
+  dst->y0 = src->y0;
+  dst->z0 = src->z0;
+  dst->kx0 = src->kx0;
+  dst->ky0 = src->ky0;
+  dst->kz0 = src->kz0;
   if (dst == src) return;
   void *s = lg_c_in_flushed(src);
   grid_dev *dd = dst->dev;

@@ -74,6 +74,57 @@ void grid_wf_free(wf *gwf) {
   free(gwf);
 }
 
+/* ---- absorbing boundary (reference src/grid_wf.c:99-120, :185-203, :728-755) --------------- */
+
+REAL grid_wf_absorb(INT i, INT j, INT k, INT lx, INT hx, INT ly, INT hy, INT lz, INT hz) {
+  REAL t = 0.0;
+  if (i < lx) t += ((REAL)(lx - i)) / (REAL)lx;
+  else if (i > hx) t += ((REAL)(i - hx)) / (REAL)lx; /* the reference divides by lx on both sides */
+  if (j < ly) t += ((REAL)(ly - j)) / (REAL)ly;
+  else if (j > hy) t += ((REAL)(j - hy)) / (REAL)ly;
+  if (k < lz) t += ((REAL)(lz - k)) / (REAL)lz;
+  else if (k > hz) t += ((REAL)(k - hz)) / (REAL)lz;
+  t *= 2.0 / 3.0; /* mean of the three directions, doubled: full imaginary time half-way in */
+  return t > 1.0 ? 1.0 : t;
+}
+
+void grid_wf_boundary(wf *gwf, wf *gwfp, INT lx, INT hx, INT ly, INT hy, INT lz, INT hz) {
+  if (!lx && !hx && !ly && !hy && !lz && !hz) {
+    gwf->ts_func = NULL;
+    gwf->lx = gwf->hx = gwf->ly = gwf->hy = gwf->lz = gwf->hz = 0;
+    if (gwfp) {
+      gwfp->ts_func = NULL;
+      gwfp->lx = 0; /* as the reference: only lx of the predict wavefunction is cleared */
+    }
+    return;
+  }
+  gwf->lx = lx;
+  gwf->hx = hx;
+  gwf->ly = ly;
+  gwf->hy = hy;
+  gwf->lz = lz;
+  gwf->hz = hz;
+  gwf->ts_func = &grid_wf_absorb;
+  if (gwfp) gwfp->ts_func = &grid_wf_absorb;
+}
+
+void grid_wf_merge(wf *dst, cgrid *wfr, cgrid *wfi) {
+  cgrid *g = dst->grid;
+  if (wfr->nx != g->nx || wfr->ny != g->ny || wfr->nz != g->nz || wfi->nx != g->nx || wfi->ny != g->ny ||
+      wfi->nz != g->nz)
+    lg_fatal("grid_wf_merge: grids differ in size");
+  const int prec = cgrid_get_precision(g);
+  if (cgrid_get_precision(wfr) != prec) cgrid_set_precision(wfr, prec);
+  if (cgrid_get_precision(wfi) != prec) cgrid_set_precision(wfi, prec);
+  const void *r = lg_c_in_flushed(wfr);
+  const void *im = lg_c_in_flushed(wfi);
+  if (g != wfr) lg_c_in_flushed(g); /* brings a pending z transform / scale to a known state */
+  void *out = lg_c_out(g);
+  const long bounds[6] = {dst->lx, dst->hx, dst->ly, dst->hy, dst->lz, dst->hz};
+  lg_hip(hgrid_wf_merge(out, r, im, g->dev->prec, g->nx, g->ny, g->nz, bounds), "wf_merge");
+  lg_c_written(g, 1);
+}
+
 void grid_wf_zero(wf *gwf) { cgrid_zero(gwf->grid); }
 void grid_wf_constant(wf *gwf, REAL complex c) { cgrid_constant(gwf->grid, c); }
 void grid_wf_map(wf *gwf, REAL complex (*func)(void *arg, REAL x, REAL y, REAL z), void *farg) {
@@ -97,7 +148,10 @@ void grid_wf_propagate_potential(wf *gwf, REAL complex tstep, cgrid *potential, 
   cgrid *g = gwf->grid;
   if (potential->nx != g->nx || potential->ny != g->ny || potential->nz != g->nz)
     lg_fatal("grid_wf_propagate_potential: potential and wavefunction grids differ in size");
-  if (gwf->ts_func) lg_fatal("grid_wf_propagate_potential: absorbing boundaries are not supported by this build");
+  /* the FFT propagators ignore ts_func here (reference src/grid_wf.c:584); the position-
+   * dependent time of the CN propagators is not built */
+  if (gwf->ts_func && gwf->propagator >= WF_2ND_ORDER_CN)
+    lg_fatal("grid_wf_propagate_potential: absorbing boundaries with the CN propagators are not supported by this build");
   if (potential->dev->prec != g->dev->prec) cgrid_set_precision(potential, cgrid_get_precision(g));
   const void *pot = lg_c_in_flushed(potential);
   const double *scale;
@@ -453,13 +507,41 @@ static void check_fft_grid(cgrid *g) {
       lg_fatal("FFT length %ld is not supported (powers of two up to 1024, 3*2^k up to 384)", (long)d[a]);
 }
 
+static void propagate_split(wf *gwf, cgrid *potential, REAL complex time, int real_space);
+
 void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
+  cgrid *g = gwf->grid;
+  check_split_propagator(gwf, "grid_wf_propagate", 1);
+  if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
+  if (gwf->ts_func && fourth_order(gwf)) lg_fatal("4th order propagator not available for FFT absorbing BC.");
+  if (gwf->ts_func && creal(time) != 0.0) {
+    /* absorbing boundary (reference src/grid_wf.c:396-422): a real-time step of psi and an
+     * imaginary-time step (time -i Re t) of a copy, merged with the boundary amplitude; pure
+     * imaginary time skips the boundary */
+    if (!gwf->cworkspace)
+      gwf->cworkspace = cgrid_alloc(g->nx, g->ny, g->nz, g->step, g->value_outside, g->outside_params_ptr, "WF cworkspace 1");
+    cgrid *w = gwf->cworkspace;
+    if (cgrid_get_precision(w) != cgrid_get_precision(g)) cgrid_set_precision(w, cgrid_get_precision(g));
+    cgrid_copy(w, g);
+    REAL (*save)(INT, INT, INT, INT, INT, INT, INT, INT, INT) = gwf->ts_func;
+    gwf->ts_func = NULL;
+    propagate_split(gwf, potential, time, 1);
+    gwf->grid = w;
+    propagate_split(gwf, potential, -I * creal(time), 1);
+    gwf->grid = g;
+    gwf->ts_func = save;
+    grid_wf_merge(gwf, g, w);
+    return;
+  }
+  propagate_split(gwf, potential, time, 0);
+}
+
+/* one split-operator step of gwf->grid by the propagator's sequence; real_space leaves the
+ * result untransformed (the absorbing-boundary merge reads it) */
+static void propagate_split(wf *gwf, cgrid *potential, REAL complex time, int real_space) {
   cgrid *g = gwf->grid;
   const REAL kx0 = g->kx0, ky0 = g->ky0, kz0 = g->kz0;
   const REAL cons = -(HBAR * HBAR / (2.0 * gwf->mass)) * (kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
-  check_split_propagator(gwf, "grid_wf_propagate", 1);
-  if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
-  if (gwf->ts_func) lg_fatal("absorbing boundaries (grid_wf_boundary) are not supported by this build");
   check_fft_grid(g);
   lg_shape s;
   lg_squeeze(g->nx, g->ny, g->nz, &s);
@@ -490,7 +572,7 @@ void grid_wf_propagate(wf *gwf, cgrid *potential, REAL complex time) {
   sa.tm_im = cimag(tm);
   sa.kcut = uses_cutoff(gwf);
   sa.kcut2 = gwf->kmax * gwf->kmax;
-  sa.no_defer = fourth; /* the last V(t/6) stage reads the real-space state */
+  sa.no_defer = fourth || real_space; /* the V(t/6) stage and the merge read the real-space state */
   fused_step(gwf, &sa);
   if (fourth) grid_wf_propagate_potential(gwf, time / 6.0, potential, cons);
 }

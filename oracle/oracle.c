@@ -367,6 +367,59 @@ void or_wf_propagate_4th(cplx *a, const cplx *pot, long nx, long ny, long nz, do
   if (pot) or_propagate_potential(a, pot, n, creal(t6), cimag(t6), cons);
 }
 
+/* grid_wf_absorb (src/grid_wf.c:185-203): the absorbing-boundary amplitude, 0 inside
+ * [lx, hx] x [ly, hy] x [lz, hz] and rising linearly outside; both sides of an axis are
+ * divided by the lower bound (lx, ly, lz), as the reference does; the sum over the three
+ * axes is multiplied by 2/3 and capped at 1. */
+double or_wf_absorb(long i, long j, long k, long lx, long hx, long ly, long hy, long lz, long hz) {
+  double t = 0.0;
+  if (i < lx) t += ((double)(lx - i)) / (double)lx;
+  else if (i > hx) t += ((double)(i - hx)) / (double)lx;
+  if (j < ly) t += ((double)(ly - j)) / (double)ly;
+  else if (j > hy) t += ((double)(j - hy)) / (double)ly;
+  if (k < lz) t += ((double)(lz - k)) / (double)lz;
+  else if (k > hz) t += ((double)(k - hz)) / (double)lz;
+  t *= 2.0 / 3.0;
+  if (t > 1.0) return 1.0;
+  return t;
+}
+
+/* grid_wf_merge (src/grid_wf.c:728-755): dst = (1 - alpha) wr + alpha wi, alpha from
+ * grid_wf_absorb; dst may be wr */
+void or_wf_merge(cplx *dst, const cplx *wr, const cplx *wi, long nx, long ny, long nz, long lx, long hx, long ly, long hy,
+                 long lz, long hz) {
+  long nxy = nx * ny;
+#pragma omp parallel for schedule(static)
+  for (long ij = 0; ij < nxy; ++ij) {
+    long i = ij / ny, j = ij % ny;
+    for (long k = 0; k < nz; ++k) {
+      double alpha = or_wf_absorb(i, j, k, lx, hx, ly, hy, lz, hz);
+      dst[ij * nz + k] = (1.0 - alpha) * wr[ij * nz + k] + alpha * wi[ij * nz + k];
+    }
+  }
+}
+
+/* grid_wf_propagate with an absorbing boundary, WF_2ND_ORDER_FFT / _CFFT and Re(time) != 0
+ * (src/grid_wf.c:396-422, :464-490): the state is propagated once with time and once (a copy)
+ * with -i Re(time), and the two are merged with grid_wf_merge.  kmax < 0: plain kinetic
+ * phase; kmax >= 0: the cut-off one. */
+void or_wf_propagate_absorb(cplx *a, const cplx *pot, long nx, long ny, long nz, double step, double mass, double kx0,
+                            double ky0, double kz0, double tre, double tim, double kmax, long lx, long hx, long ly,
+                            long hy, long lz, long hz) {
+  long n = nx * ny * nz;
+  cplx *w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  memcpy(w, a, sizeof(cplx) * (size_t)n);
+  if (kmax < 0.0) {
+    or_wf_propagate(a, pot, nx, ny, nz, step, mass, kx0, ky0, kz0, tre, tim);
+    or_wf_propagate(w, pot, nx, ny, nz, step, mass, kx0, ky0, kz0, 0.0, -tre);
+  } else {
+    or_wf_propagate_cfft(a, pot, nx, ny, nz, step, mass, kx0, ky0, kz0, tre, tim, kmax);
+    or_wf_propagate_cfft(w, pot, nx, ny, nz, step, mass, kx0, ky0, kz0, 0.0, -tre, kmax);
+  }
+  or_wf_merge(a, a, w, nx, ny, nz, lx, hx, ly, hy, lz, hz);
+  free(w);
+}
+
 /* grid_wf_normalize (src/grid_wf.c:694-700): multiply by sqrt(target / norm); returns norm */
 double or_wf_normalize(cplx *a, long nx, long ny, long nz, double step, double target) {
   double norm = or_integral_of_square(a, nx, ny, nz, step);

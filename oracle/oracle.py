@@ -38,6 +38,9 @@ _PROTOS = {
     "or_propagate_kinetic_cfft": (None, [P, L, L, L, D, D, D, D, D, D, D, D]),
     "or_wf_propagate_cfft": (None, [P, P, L, L, L, D, D, D, D, D, D, D, D]),
     "or_wf_propagate_4th": (None, [P, P, L, L, L, D, D, D, D, D, D, D, D]),
+    "or_wf_absorb": (D, [L, L, L, L, L, L, L, L, L]),
+    "or_wf_merge": (None, [P, P, P, L, L, L, L, L, L, L, L, L]),
+    "or_wf_propagate_absorb": (None, [P, P, L, L, L, D, D, D, D, D, D, D, D, L, L, L, L, L, L]),
     "or_wf_normalize": (D, [P, L, L, L, D, D]),
     "or_kinetic_energy_fft": (D, [P, L, L, L, D, D]),
     "or_energy_fft": (D, [P, P, L, L, L, D, D]),
@@ -171,6 +174,27 @@ def wf_propagate_4th_inplace(b: np.ndarray, pot, step, mass, k0, time, kmax=None
     pp = None if pot is None else _c(pot).ctypes.data
     lib().or_wf_propagate_4th(_c(b).ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag,
                               -1.0 if kmax is None else kmax)
+
+
+def wf_absorb(i, j, k, bounds) -> float:
+    """grid_wf_absorb at (i, j, k); bounds = (lx, hx, ly, hy, lz, hz)."""
+    return lib().or_wf_absorb(i, j, k, *bounds)
+
+
+def wf_merge(wr, wi, bounds) -> np.ndarray:
+    wr, wi = _c(wr), _c(wi)
+    out = np.empty_like(wr)
+    lib().or_wf_merge(out.ctypes.data, wr.ctypes.data, wi.ctypes.data, *wr.shape, *bounds)
+    return out
+
+
+def wf_propagate_absorb_inplace(b: np.ndarray, pot, step, mass, k0, time, bounds, kmax=None) -> None:
+    """One reference grid_wf_propagate step with an absorbing boundary (2nd order; CFFT when
+    kmax is given) in place; pot may be None."""
+    t = complex(time)
+    pp = None if pot is None else _c(pot).ctypes.data
+    lib().or_wf_propagate_absorb(_c(b).ctypes.data, pp, *b.shape, step, mass, k0[0], k0[1], k0[2], t.real, t.imag,
+                                 -1.0 if kmax is None else kmax, *bounds)
 
 
 def wf_normalize(a, step, target=1.0):

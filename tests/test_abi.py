@@ -90,3 +90,14 @@ def test_no_gpu_means_loud_failure():
                        env=dict(os.environ, HIP_VISIBLE_DEVICES="", PYTHONPATH=ROOT))
     assert p.returncode != 0
     assert "libgrid:" in p.stderr
+
+
+def test_absorb_amplitude_matches_oracle(orc):
+    """grid_wf_absorb is host arithmetic (no GPU call): every point of a 12 x 9 x 10 grid for
+    several boundary boxes, bit for bit against the oracle's restatement."""
+    lib = libgrid_amd.load()
+    for b in [(3, 8, 2, 6, 2, 7), (1, 10, 4, 4, 3, 5), (5, 6, 1, 7, 4, 5)]:
+        for i in range(12):
+            for j in range(9):
+                for k in range(10):
+                    assert lib.grid_wf_absorb(i, j, k, *b) == orc.wf_absorb(i, j, k, b), (b, i, j, k)

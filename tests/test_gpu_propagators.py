@@ -6,7 +6,9 @@
     so the set of removed components is the oracle's exactly.  kmax is chosen so that a
     sizeable share of the spectrum is removed.
   * WF_4TH_ORDER_FFT / WF_4TH_ORDER_CFFT (src/grid_wf.c:433-457, :505-525):
-    V(t/6) K(t/2) V(2t/3) K(t/2) V(t/6).
+    V(t/6) K(t/2) V(2t/3) K(t/2) V(t/6);
+  * the absorbing boundary of the 2nd-order ones (grid_wf_boundary, src/grid_wf.c:396-422):
+    a real-time and an imaginary-time step merged by the boundary amplitude.
 
 The bar is the FP64 one of the plain step (relative L2 <= 1e-12).
 """
@@ -179,5 +181,48 @@ def test_fourth_order_after_second_order_steps(lg, orc):
         lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
         orc.wf_propagate_4th_inplace(ref, pot, step, mass, (0, 0, 0), time)
     assert rel_l2(LG.get_cgrid(w.contents.grid), ref) < TOL
+    lg.grid_wf_free(w)
+    lg.cgrid_free(vp)
+
+
+@pytest.mark.parametrize("shape,k0,time,frac,bounds", [
+    ((32, 32, 32), (0, 0, 0), 0.05, None, (6, 25, 5, 26, 4, 27)),
+    ((16, 32, 64), (0.0, 0.3, -0.2), 0.04 - 0.01j, None, (3, 12, 6, 25, 10, 53)),
+    ((32, 32, 32), (0.1, 0, 0), 0.05, 0.6, (6, 25, 5, 26, 4, 27)),
+    ((1, 64, 64), (0, 0, 0), 0.05, None, (0, 0, 8, 55, 8, 55)),
+    ((1, 1, 256), (0, 0, 0.2), 0.05, 0.5, (0, 0, 0, 0, 32, 223)),
+])
+def test_absorbing_boundary_matches_oracle(lg, orc, shape, k0, time, frac, bounds):
+    """grid_wf_boundary + grid_wf_propagate (reference src/grid_wf.c:396-422, :464-490)."""
+    case(f"absorbing {shape} k0={k0} t={time} cutoff={frac} bounds={bounds}")
+    step, mass = 0.2, 1.0
+    kmax = None if frac is None else kmax_for(shape, step, frac)
+    prop = LG.WF_2ND_ORDER_FFT if frac is None else LG.WF_2ND_ORDER_CFFT
+    rng = np.random.default_rng(6)
+    psi = W.normalize(rand_c(shape, 14) * 0.1 + 1.0, step)
+    pot = (0.5 * rng.uniform(0, 1, shape) + 0.01j * rng.uniform(0, 1, shape)).astype(np.complex128)
+    w = new_wf(lg, psi, step, mass, kmax, prop)
+    lg.cgrid_set_momentum(w.contents.grid, *k0)
+    lg.grid_wf_boundary(w, None, *bounds)
+    vp = lg.cgrid_alloc(*shape, step, LG.boundary("cgrid_value_outside_periodic"), None, b"pot")
+    LG.set_cgrid(vp, pot)
+    ref = psi.copy()
+    for _ in range(3):
+        lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
+        orc.wf_propagate_absorb_inplace(ref, pot, step, mass, k0, time, bounds, kmax)
+    out = LG.get_cgrid(w.contents.grid)
+    assert rel_l2(out, ref) < TOL
+    # the boundary did act: the plain step differs
+    plain = psi.copy()
+    for _ in range(3):
+        orc.wf_propagate_inplace(plain, pot, step, mass, k0, time)
+    assert rel_l2(plain, ref) > 1e-6
+    # pure imaginary time skips the boundary
+    lg.grid_wf_propagate(w, vp, LG.c_complex.of(-0.05j))
+    orc.wf_propagate_inplace(ref, pot, step, mass, k0, -0.05j) if kmax is None else \
+        orc.wf_propagate_cfft_inplace(ref, pot, step, mass, k0, -0.05j, kmax)
+    assert rel_l2(LG.get_cgrid(w.contents.grid), ref) < TOL
+    lg.grid_wf_boundary(w, None, 0, 0, 0, 0, 0, 0)
+    assert not w.contents.ts_func
     lg.grid_wf_free(w)
     lg.cgrid_free(vp)

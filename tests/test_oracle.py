@@ -263,6 +263,51 @@ def test_4th_order_limits_and_reversibility(orc):
     assert rel_l2(b, psi) < 1e-13
 
 
+def test_absorb_amplitude_known_values(orc):
+    """grid_wf_absorb (src/grid_wf.c:185-203) at hand-computed points: 0 inside the box
+    [lx, hx] x [ly, hy] x [lz, hz]; outside, (distance / lower bound) per axis, summed, times
+    2/3, capped at 1.  The upper side divides by the lower bound too (lx, not nx - 1 - hx)."""
+    b = (4, 11, 2, 5, 3, 12)
+    cases = {(6, 3, 7): 0.0, (4, 2, 3): 0.0, (11, 5, 12): 0.0, (0, 3, 7): 2 / 3, (2, 3, 7): 1 / 3,
+             (13, 3, 7): 1 / 3, (15, 3, 7): 2 / 3, (6, 0, 7): 2 / 3, (6, 7, 7): 2 / 3, (6, 3, 0): 2 / 3,
+             (6, 3, 14): 4 / 9, (2, 6, 7): (0.5 + 0.5) * 2 / 3, (0, 0, 0): 1.0, (15, 7, 15): 1.0}
+    for (i, j, k), want in cases.items():
+        assert abs(orc.wf_absorb(i, j, k, b) - want) < 1e-15, (i, j, k)
+
+
+def test_absorbing_step_regions_and_absorption(orc):
+    """With an absorbing boundary the step (src/grid_wf.c:396-422) is the real-time step where
+    the amplitude is 0 and the imaginary-time step (time -i Re t) where it is 1, bit for bit; a
+    packet that runs into the absorbing layer loses its norm, while without the boundary the
+    same packet keeps it."""
+    rng = np.random.default_rng(3)
+    shape, step, mass, t = (16, 8, 16), 0.3, 1.0, 0.05
+    b = (4, 11, 2, 5, 3, 12)
+    psi = np.ascontiguousarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    pot = np.ascontiguousarray(rng.uniform(0, 1, shape) + 0j)
+    ab, re_, im_ = psi.copy(), psi.copy(), psi.copy()
+    orc.wf_propagate_absorb_inplace(ab, pot, step, mass, (0, 0, 0), t, b)
+    orc.wf_propagate_inplace(re_, pot, step, mass, (0, 0, 0), t)
+    orc.wf_propagate_inplace(im_, pot, step, mass, (0, 0, 0), -1j * t)
+    alpha = np.array([[[orc.wf_absorb(i, j, k, b) for k in range(shape[2])] for j in range(shape[1])]
+                      for i in range(shape[0])])
+    assert np.array_equal(ab[alpha == 0], re_[alpha == 0]) and np.any(alpha == 0)
+    assert np.array_equal(ab[alpha == 1], im_[alpha == 1]) and np.any(alpha == 1)
+    mid = (alpha > 0) & (alpha < 1)
+    assert rel_l2(ab[mid], ((1 - alpha) * re_ + alpha * im_)[mid]) < 1e-15
+    # a packet with momentum +3 along z runs into the layer of indices k > 191 (and k < 64)
+    n, step, mass, p = 256, 0.1, 1.0, 3.0
+    _, _, z = W.coords((1, 1, n), step)
+    psi = W.normalize(np.ascontiguousarray(np.exp(-((z - 2.0) ** 2) + 1j * p * z) * np.ones((1, 1, n))), step)
+    free, ab = psi.copy(), psi.copy()
+    bz = (0, 0, 0, 0, 64, 191)
+    for _ in range(100):  # the centre moves p t / m = 9 bohr, from 4.3 before the layer into it
+        orc.wf_propagate_absorb_inplace(ab, None, step, mass, (0, 0, 0), 0.03, bz)
+        orc.wf_propagate_inplace(free, None, step, mass, (0, 0, 0), 0.03)
+    assert abs(orc.integral_of_square(free, step) - 1.0) < 1e-12
+    assert orc.integral_of_square(ab, step) < 0.25
+
+
 def test_constant_and_absorbing_potential(orc):
     """A constant potential V0 is a global phase exp(-i V0 t); V0 - i G also decays as exp(-G t)."""
     rng = np.random.default_rng(5)
