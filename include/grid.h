@@ -252,7 +252,7 @@ void grid_wf_analyze_method(char method);                                       
  * One 2nd-order FFT step of i dpsi/dt = (T + vext + g|psi|^2) psi with the nonlinear
  * potential built from the state at the start of the step, exactly as
  *   grid_wf_density -> (g * rho + vext) -> grid_real_to_complex_re -> grid_wf_propagate
- * would, followed by grid_wf_normalize when normalize != 0 — fused into 9 FFT passes.  It
+ * would, followed by grid_wf_normalize when normalize != 0 — fused into 8 or 9 FFT passes.  It
  * also records E_GP = T + int vext|psi|^2 + (g/2) int |psi|^4 of the start-of-step state. */
 void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g, REAL complex time, char normalize);
 /* E_GP of the state that started the most recent grid_wf_gpe_propagate call. */
@@ -260,10 +260,13 @@ REAL grid_wf_gpe_last_energy(wf *gwf);
 /* E_GP of the current state (FFT kinetic energy, no k0 shift, as grid_wf_kinetic_energy_fft). */
 REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g);
 /* Imaginary-time relaxation with renormalisation every step until the energy is converged to
- * tol, or max_steps.  Converged: with energies sampled every m steps (m doubling from 4 while
- * the decay is slow) the last decrease D2 is below tol and so is the geometric estimate
- * D2 q / (1 - q), q = D2 / D1, of the energy still to come; a rise smaller than tol (rounding)
- * also ends it.  Returns the number of steps taken and stores the last energy in *energy. */
+ * tol, or max_steps.  Converged: with energies sampled every m steps (m = 16, doubling up to
+ * 4096 while the decay is slow) the last decrease D2 is below tol and so is the geometric
+ * estimate D2 q / (1 - q), q = D2 / D1, of the energy still to come, with two successive
+ * ratios q within a factor of two; a rise smaller than tol (rounding) also ends it.  Steps run
+ * in batches of 16 and only the first step of a batch, where every checkpoint falls, forms the
+ * energy.  Returns the number of steps up to the deciding checkpoint (the state has run to the
+ * end of its batch) and stores the energy of the last checkpoint in *energy. */
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_steps, REAL *energy);
 /* Relaxation replays batches of 16 steps as a HIP graph after the first batch (default on);
  * grid_hip_graphs(0) launches every step eagerly instead. */

@@ -74,8 +74,9 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
 #define HGRID_PF_MID_KIN    0x0008u  /* mid: psi *= ktab_a[n] * ktab_1[c1] * ktab_2[c2] (double2 tables) */
 #define HGRID_PF_MID_KRED   0x0010u  /* mid: partial sum of k^2 |psi_k|^2 (before MID_KIN) */
 #define HGRID_PF_MID_POT    0x0020u  /* mid: psi *= mid * exp(c * (cons + pot)), complex pot */
-#define HGRID_PF_MID_GPE    0x0040u  /* mid: psi *= mid * exp(c * (cons + vext + g * rho)), or
-                                        * exp(c * (cons + rho)) when vext is NULL (rho holds V_eff) */
+#define HGRID_PF_MID_GPE    0x0040u  /* mid: psi *= mid * exp(c * (cons + vext + g * rho)),
+                                        * exp(c * (cons + rho)) when vext is NULL (rho holds V_eff),
+                                        * or exp(c * (cons + vext)) when rho is NULL */
 #define HGRID_PF_POST_RED   0x0080u  /* post: partial sum of |psi|^2 of the stored values */
 #define HGRID_PF_NOSTORE    0x0100u  /* reduce only; do not write the array */
 #define HGRID_PF_PRE_VEFF   0x0200u  /* pre: rho_out = vext + g |psi|^2 (GPE potential) instead of |psi|^2 */

@@ -297,8 +297,8 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
   } else if constexpr (KIND == KIND_POT || KIND == KIND_GPE) {
     const Rl mid = (Rl)a.mid;
     const auto rpot = make_rsrc(a.pot, (unsigned long)ntot * CS);
-    const auto rvx = make_rsrc(a.vext, (unsigned long)ntot * 8);
-    const auto rrho = make_rsrc(a.rho, (unsigned long)ntot * 8);
+    const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
+    const auto rrho = make_rsrc(a.rho, a.rho ? (unsigned long)ntot * 8 : 0);
     if constexpr (KIND == KIND_POT) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
@@ -306,14 +306,17 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
         v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, a.cons + (double)pv.x, (double)pv.y));
       }
     } else {
-      // real potential: V = vext + g rho, or rho alone when it already holds V_eff
+      // real potential: V = vext + g rho, rho alone when it already holds V_eff, or vext alone
       double vr[R];
-      if (a.vext) {
+      if (a.vext && a.rho) {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
           const unsigned e = e0 + (unsigned)j * estep;
           vr[j] = a.cons + bload_f64(rvx, e * 8u, 0) + a.g * bload_f64(rrho, e * 8u, 0);
         }
+      } else if (a.vext) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) vr[j] = a.cons + bload_f64(rvx, (e0 + (unsigned)j * estep) * 8u, 0);
       } else {
 #pragma unroll
         for (int j = 0; j < R; ++j) vr[j] = a.cons + bload_f64(rrho, (e0 + (unsigned)j * estep) * 8u, 0);

@@ -315,7 +315,10 @@ static void fused_step(wf *gwf, const step_args *sa) {
   const int zin = d->zpend;
   ktab_cache *kt = lg_kinetic_tables(g, sa->tm_re, sa->tm_im, g->fft_norm);
   ktab_cache *k2 = (sa->want_energy) ? lg_k2_tables(g) : NULL;
-  if (sa->gpe && !d->rho) lg_hip(hgrid_malloc((void **)&d->rho, d->n * sizeof(double)), "hipMalloc (density)");
+  /* V_eff = vext + g |psi|^2 is materialised only for g != 0; with g = 0 the potential pass
+   * reads vext itself */
+  const int veff = sa->gpe && sa->g != 0.0;
+  if (veff && !d->rho) lg_hip(hgrid_malloc((void **)&d->rho, d->n * sizeof(double)), "hipMalloc (density)");
 
   const long nb2 = hgrid_pass_blocks(prec, s.n[0], s.n[1], s.n[2], 2);
   const long nbK = hgrid_pass_blocks(prec, s.n[0], s.n[1], s.n[2], aK);
@@ -330,7 +333,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
    * the density / reductions and stores nothing; the first y pass then applies the scale. */
   lg_pass_init(&p, &s, prec, psi, psi, 2);
   if (scale) { p.flags |= HGRID_PF_PRESCALE; p.scale = scale; }
-  if (sa->gpe) {
+  if (veff) {
     /* V_eff = vext + g |psi|^2 of the start-of-step state, read back by the potential pass */
     p.flags |= HGRID_PF_PRE_VEFF;
     p.rho_out = d->rho;
@@ -341,7 +344,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
   if (zin) {
     p.dir1 = +1;
     p.flags |= HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE;
-    if (sa->gpe || sa->want_energy) lg_run_pass(&p);
+    if (veff || sa->want_energy) lg_run_pass(&p); /* otherwise nothing reads the input state */
   } else {
     lg_run_pass(&p);
   }
@@ -413,10 +416,14 @@ static void fused_step(wf *gwf, const step_args *sa) {
         p.c_im = sa->c_im;
         p.cons = sa->cons;
         p.mid = 1.0;
-        if (sa->gpe) {
+        if (veff) {
           p.flags |= HGRID_PF_MID_GPE;
           p.vext = NULL; /* d->rho holds V_eff */
           p.rho = d->rho;
+        } else if (sa->gpe) {
+          p.flags |= HGRID_PF_MID_GPE;
+          p.vext = sa->vext; /* g = 0: V = vext */
+          p.rho = NULL;
         } else {
           p.flags |= HGRID_PF_MID_POT;
           p.pot = sa->pot;
@@ -673,7 +680,15 @@ static double gpe_energy_from_parts(wf *gwf, const double *parts, double g_nl) {
 
 static double g_last_g = 0.0;
 
+static void gpe_step(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize, int want_energy);
+
 void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize) {
+  gpe_step(gwf, vext, g_nl, time, normalize, 1);
+}
+
+/* one fused GPE step; without want_energy the energy reductions are left out, and with
+ * g = 0 a deferred 3-D state is not read back at all before the transforms */
+static void gpe_step(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize, int want_energy) {
   cgrid *g = gwf->grid;
   check_split_propagator(gwf, "grid_wf_gpe_propagate", 0);
   if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
@@ -696,12 +711,12 @@ void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, c
   sa.tm_im = cimag(tm);
   sa.kcut = uses_cutoff(gwf);
   sa.kcut2 = gwf->kmax * gwf->kmax;
-  sa.want_energy = 1;
+  sa.want_energy = want_energy;
   sa.want_post_norm = normalize ? 1 : 0;
   sa.renorm_target = normalize ? gwf->norm : 0.0;
   fused_step(gwf, &sa);
   grid_dev *d = g->dev;
-  d->have_gpe_energy = 1;
+  d->have_gpe_energy = want_energy;
   g_last_g = g_nl;
 }
 
@@ -737,10 +752,13 @@ void grid_hip_graphs(int enable) { g_use_graphs = enable ? 1 : 0; }
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy) {
   grid_dev *d = gwf->grid->dev;
   const INT batch = 16;
+#if LG_RELAX_MIN_STRIDE % 16 != 0
+#error "the relaxation batches assume stopping-rule strides that are multiples of 16"
+#endif
   double *hist = NULL;
   lg_ensure_device();
   lg_hip(hgrid_malloc((void **)&hist, sizeof(double) * HGRID_RED_NCHAN * batch), "hipMalloc (energy history)");
-  double host[HGRID_RED_NCHAN * 16];
+  double host[HGRID_RED_NCHAN];
   double e = NAN;
   INT done = -1, n = 0;
   lg_relax_state rs;
@@ -757,24 +775,21 @@ INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
       const int capture = g_use_graphs && !graph && n > 0 && nb == batch;
       if (capture) lg_hip(hgrid_graph_begin(), "hipStreamBeginCapture (relaxation)");
       for (INT b = 0; b < nb; ++b) {
-        grid_wf_gpe_propagate(gwf, vext, g_nl, -I * dt, 1);
-        lg_hip(hgrid_memcpy_d2d(hist + HGRID_RED_NCHAN * b, d->dscal + GS_RED, sizeof(double) * HGRID_RED_NCHAN),
-               "hipMemcpy D2D (energy history)");
+        /* the rule's checkpoints fall on multiples of its strides, all multiples of 16 from
+         * step 0, so only the first step of a batch computes its input state's energy */
+        gpe_step(gwf, vext, g_nl, -I * dt, 1, b == 0);
+        if (b == 0)
+          lg_hip(hgrid_memcpy_d2d(hist, d->dscal + GS_RED, sizeof(double) * HGRID_RED_NCHAN),
+                 "hipMemcpy D2D (energy history)");
       }
       if (capture) {
         lg_hip(hgrid_graph_end(&graph), "hipStreamEndCapture (relaxation)");
         lg_hip(hgrid_graph_launch(graph), "hipGraphLaunch (relaxation)"); /* capture does not execute */
       }
     }
-    lg_hip(hgrid_memcpy_d2h(host, hist, sizeof(double) * HGRID_RED_NCHAN * nb), "hipMemcpy D2H (energy history)");
-    for (INT b = 0; b < nb; ++b) {
-      e = gpe_energy_from_parts(gwf, host + HGRID_RED_NCHAN * b, g_nl);
-      /* host[b] is the energy of the state that entered step n + b + 1 */
-      if (lg_relax_feed(&rs, (long)(n + b), e, tol)) {
-        done = n + b + 1;
-        break;
-      }
-    }
+    lg_hip(hgrid_memcpy_d2h(host, hist, sizeof(double) * HGRID_RED_NCHAN), "hipMemcpy D2H (energy history)");
+    e = gpe_energy_from_parts(gwf, host, g_nl); /* the energy of the state entering step n */
+    if (lg_relax_feed(&rs, (long)n, e, tol)) done = n + 1;
     n += nb;
   }
   if (graph) lg_hip(hgrid_graph_free(graph), "hipGraphExecDestroy");
