@@ -155,6 +155,10 @@ class hgrid_pass(C.Structure):
         ("k2_a", C.c_void_p),
         ("k2_1", C.c_void_p),
         ("k2_2", C.c_void_p),
+        ("kc_a", C.c_void_p),
+        ("kc_1", C.c_void_p),
+        ("kc_2", C.c_void_p),
+        ("kcut2", C.c_double),
         ("pot", C.c_void_p),
         ("vext", C.c_void_p),
         ("rho", C.c_void_p),

@@ -306,11 +306,10 @@ static void fused_step(wf *gwf, const step_args *sa) {
   const int prec = d->prec, aK = 3 - s.m;
   /* 3-D: the state stays transformed along the contiguous axis between steps (zpend); the
    * step's first forward and last inverse transform along that axis cancel, and a read-only
-   * pass recovers the real-space quantities of the start-of-step state instead.  Only for
-   * FP64 arithmetic: with FP32 butterflies, imaginary-time relaxation from a transformed state
-   * needed about four times the steps to meet the stopping rule, so FP32 grids keep the
-   * real-space state between steps. */
-  const int defer = s.m == 3 && prec != HGRID_C64 && !sa->no_defer;
+   * pass recovers the real-space quantities of the start-of-step state instead (skipped when
+   * nothing needs them).  With FP32 butterflies the two roundings fewer per step also lower
+   * the error against FP64 (512^3 vortex ring after 500 steps: 1.3e-4 instead of 1.6e-4). */
+  const int defer = s.m == 3 && !sa->no_defer;
   if (d->zpend && !defer) lg_c_materialize(g);
   const int zin = d->zpend;
   ktab_cache *kt = lg_kinetic_tables(g, sa->tm_re, sa->tm_im, g->fft_norm);

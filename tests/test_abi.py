@@ -101,3 +101,29 @@ def test_absorb_amplitude_matches_oracle(orc):
             for j in range(9):
                 for k in range(10):
                     assert lib.grid_wf_absorb(i, j, k, *b) == orc.wf_absorb(i, j, k, b), (b, i, j, k)
+
+
+def test_every_struct_mirror_matches_c_field_by_field():
+    """sizeof and the offset of every field of each ctypes mirror (cgrid, rgrid, wf, hgrid_pass)
+    against the C compiler's, so a field added on one side only fails here, on the CPU."""
+    mirrors = {"cgrid": libgrid_amd.cgrid, "rgrid": libgrid_amd.rgrid, "wf": libgrid_amd.wf,
+               "hgrid_pass": libgrid_amd.hgrid_pass}
+    lines = []
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            field = f[0].rstrip("_")
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {field}));')
+    src = "#include <stddef.h>\n#include <stdio.h>\n#include \"grid.h\"\n#include \"grid_hip.h\"\n" \
+          "int main(void) {\n" + "\n".join(lines) + "\n  return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=gnu11", "-I" + os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines()
+    for line in out:
+        cname, field, val = line.split()
+        py = mirrors[cname]
+        want = C.sizeof(py) if field == "sizeof" else getattr(py, field).offset
+        assert int(val) == want, (cname, field, int(val), want)
