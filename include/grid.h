@@ -182,6 +182,7 @@ void cgrid_zero(cgrid *grid);
 void cgrid_constant(cgrid *grid, REAL complex c);
 void cgrid_copy(cgrid *dst, cgrid *src);
 void cgrid_multiply(cgrid *grid, REAL complex c);                              /* :1223 */
+void cgrid_add_scaled(cgrid *gridc, REAL complex d, cgrid *grida);              /* :1303 */
 REAL cgrid_integral_of_square(cgrid *grid);                                     /* :1610 */
 REAL complex cgrid_integral_of_conjugate_product(cgrid *grida, cgrid *gridb);   /* :1640 */
 void cgrid_fft(cgrid *grid);                                                    /* :1822 */
@@ -268,6 +269,15 @@ REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g);
  * energy.  Returns the number of steps up to the deciding checkpoint (the state has run to the
  * end of its batch) and stores the energy of the last checkpoint in *energy. */
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_steps, REAL *energy);
+/* grid_wf_gpe_relax with vector extrapolation (this build's addition; no reference
+ * counterpart): every `stride` steps (a multiple of 16) a snapshot is taken, and when the
+ * differences of three successive snapshots are parallel, d2 = r d1 with 0.05 < r < 0.995, the
+ * geometric tail d2 r / (1 - r) of that slowest mode is added and the state renormalised.  The
+ * stopping rule restarts after each extrapolation, so convergence is still decided by plain
+ * steps.  Uses three grids of workspace.  *n_extrap (may be NULL) receives the number of
+ * extrapolations. */
+INT grid_wf_gpe_relax_extrapolated(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_steps, REAL *energy,
+                                   INT stride, INT *n_extrap);
 /* Relaxation replays batches of 16 steps as a HIP graph after the first batch (default on);
  * grid_hip_graphs(0) launches every step eagerly instead. */
 void grid_hip_graphs(int enable);

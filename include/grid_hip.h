@@ -184,6 +184,8 @@ int hgrid_kinetic_phase(void *psi, int prec, long nx, long ny, long nz, const vo
 /* dst = (1 - alpha) wr + alpha wi with the absorbing-boundary amplitude alpha(i, j, k) of the
  * box bounds = {lx, hx, ly, hy, lz, hz} (host array); dst may be wr */
 int hgrid_wf_merge(void *dst, const void *wr, const void *wi, int prec, long nx, long ny, long nz, const long *bounds);
+/* c += (dr + i di) a, n elements of precision prec */
+int hgrid_caxpy(void *c, const void *a, int prec, long n, double dr, double di);
 /* psi *= (c_re + i c_im) * (*scale if scale) */
 int hgrid_cscale(void *psi, int prec, long n, double c_re, double c_im, const double *scale);
 /* dst = |src|^2 (dense FP64 output) */

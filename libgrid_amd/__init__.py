@@ -228,6 +228,7 @@ PROTOTYPES = {
     "cgrid_value_at_index": (CX, [CG, L, L, L]),
     "cgrid_value_to_index": (V, [CG, L, L, L, CX]),
     "cgrid_fft_laplace_expectation_value": (D, [CG]),
+    "cgrid_add_scaled": (V, [CG, CX, CG]),
     "cgrid_set_precision": (V, [CG, CHR]),
     "cgrid_get_precision": (CHR, [CG]),
     "cgrid_host_sync": (V, [CG]),
@@ -274,6 +275,7 @@ PROTOTYPES = {
     "grid_wf_gpe_last_energy": (D, [WF]),
     "grid_wf_gpe_energy": (D, [WF, RG, D]),
     "grid_wf_gpe_relax": (L, [WF, RG, D, D, D, L, C.POINTER(D)]),
+    "grid_wf_gpe_relax_extrapolated": (L, [WF, RG, D, D, D, L, C.POINTER(C.c_double), L, C.POINTER(L)]),
     "grid_wf_batch2d_relax": (D, [L, C.POINTER(WF), C.POINTER(RG), D, D, L, C.POINTER(D), C.POINTER(L)]),
     "grid_batch2d_autotune": (C.c_char_p, [L, C.POINTER(WF), C.POINTER(RG), D, L]),
     "grid_batch2d_solve_times": (L, [C.POINTER(D), C.POINTER(D)]),

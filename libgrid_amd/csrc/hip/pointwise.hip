@@ -314,6 +314,16 @@ __global__ void k_wf_merge(C *dst, const C *wr, const C *wi, long nx, long ny, l
   }
 }
 
+// c += d a (complex d formed in FP64, product and sum rounded once to the storage precision)
+template <typename C> __global__ void k_caxpy(C *c, const C *a, long n, double dr, double di) {
+  LG_GRID_STRIDE(i, n) {
+    const C x = a[i], y = c[i];
+    const double re = (double)y.x + (dr * (double)x.x - di * (double)x.y);
+    const double im = (double)y.y + (dr * (double)x.y + di * (double)x.x);
+    c[i] = mk<C>((typename real_of<C>::type)re, (typename real_of<C>::type)im);
+  }
+}
+
 template <typename C> __global__ void k_cscale(C *psi, long n, double cr, double ci, const double *scale) {
   C f = mk<C>((typename real_of<C>::type)cr, (typename real_of<C>::type)ci);
   if (scale) {
@@ -516,6 +526,11 @@ extern "C" int hgrid_wf_merge(void *dst, const void *wr, const void *wi, int pre
               bounds[1], bounds[2], bounds[3], bounds[4], bounds[5]);
   LG_LAUNCH(k_wf_merge<float2>, n, (float2 *)dst, (const float2 *)wr, (const float2 *)wi, nx, ny, nz, bounds[0],
             bounds[1], bounds[2], bounds[3], bounds[4], bounds[5]);
+}
+
+extern "C" int hgrid_caxpy(void *c, const void *a, int prec, long n, double dr, double di) {
+  if (prec == HGRID_C128) LG_LAUNCH(k_caxpy<double2>, n, (double2 *)c, (const double2 *)a, n, dr, di);
+  LG_LAUNCH(k_caxpy<float2>, n, (float2 *)c, (const float2 *)a, n, dr, di);
 }
 
 extern "C" int hgrid_cscale(void *psi, int prec, long n, double cr, double ci, const double *scale) {

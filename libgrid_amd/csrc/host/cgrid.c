@@ -419,6 +419,17 @@ void cgrid_copy(cgrid *dst, cgrid *src) {
 
 void cgrid_multiply(cgrid *grid, REAL complex c) { lg_c_set_pending(grid, creal(c), cimag(c)); }
 
+/* gridc = gridc + d * grida (reference src/cgrid.c:1303-1319) */
+void cgrid_add_scaled(cgrid *gridc, REAL complex d, cgrid *grida) {
+  if (gridc->nx != grida->nx || gridc->ny != grida->ny || gridc->nz != grida->nz)
+    lg_fatal("Incompatible grids in cgrid_add_scaled().");
+  if (cgrid_get_precision(grida) != cgrid_get_precision(gridc)) cgrid_set_precision(grida, cgrid_get_precision(gridc));
+  const void *a = lg_c_in_flushed(grida);
+  void *c = lg_c_in_flushed(gridc);
+  lg_hip(hgrid_caxpy(c, a, gridc->dev->prec, (long)gridc->dev->n, creal(d), cimag(d)), "caxpy");
+  lg_c_written(gridc, 1);
+}
+
 static double fetch_scalar(const double *dptr) {
   double v;
   lg_hip(hgrid_memcpy_d2h(&v, dptr, sizeof(double)), "hipMemcpy D2H (scalar)");

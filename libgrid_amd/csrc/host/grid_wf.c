@@ -748,34 +748,110 @@ static int g_use_graphs = 1;
 
 void grid_hip_graphs(int enable) { g_use_graphs = enable ? 1 : 0; }
 
-INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy) {
+/* Vector extrapolation of the relaxation (used when stride > 0).  Snapshots psi_k are taken
+ * every `stride` steps.  With the differences d1 = psi_k - psi_{k-1} and d2 = psi_{k+1} - psi_k,
+ * a slowest mode decaying by r per stride shows as d2 = r d1.  When d1 and d2 are parallel
+ * (|<d1|d2>| > LG_EXTRAP_COS |d1| |d2|) and r = Re<d1|d2> / |d1|^2 lies in (0.05, 0.995), the
+ * mode's remaining geometric tail d2 r / (1 - r) is added at once (vector Aitken), the state is
+ * renormalised and the stopping rule restarts (for FP64 grids without its small-rise exit,
+ * so a tail that an overshoot left rising is followed to the end).  The fixed point is
+ * unchanged; the plain rule still decides convergence. */
+#define LG_EXTRAP_COS 0.999
+
+typedef struct {
+  void *s1, *d1, *d2;  /* device arrays of the grid's precision: last snapshot, last difference, scratch */
+  int have;            /* snapshots since the last extrapolation (0, 1, 2+) */
+  long count;          /* extrapolations done */
+} extrap_state;
+
+static double complex extrap_dot(cgrid *g, const void *a, const void *b) {
+  grid_dev *d = g->dev;
+  double *scr = lg_partials(hgrid_reduce_scratch((long)d->n));
+  lg_hip(hgrid_sum_conj_product(d->dscal + GS_TMP, a, b, d->prec, (long)d->n, scr), "sum_conj_product");
+  double v[2];
+  lg_hip(hgrid_memcpy_d2h(v, d->dscal + GS_TMP, sizeof(v)), "hipMemcpy D2H (scalar)");
+  return v[0] + I * v[1];
+}
+
+/* returns 1 if it changed the state; *r_out receives the ratio per stride it used */
+static int extrap_snapshot(wf *gwf, extrap_state *x, double *r_out) {
+  cgrid *g = gwf->grid;
+  grid_dev *d = g->dev;
+  const size_t bytes = d->n * lg_prec_size(d->prec);
+  const long n = (long)d->n;
+  if (!x->s1) {
+    lg_hip(hgrid_malloc(&x->s1, bytes), "hipMalloc (relax snapshot)");
+    lg_hip(hgrid_malloc(&x->d1, bytes), "hipMalloc (relax difference)");
+    lg_hip(hgrid_malloc(&x->d2, bytes), "hipMalloc (relax difference)");
+  }
+  void *psi = lg_c_in_flushed(g); /* real space, no pending scale */
+  if (x->have == 0) {
+    lg_hip(hgrid_memcpy_d2d(x->s1, psi, bytes), "hipMemcpy D2D (snapshot)");
+    x->have = 1;
+    return 0;
+  }
+  lg_hip(hgrid_memcpy_d2d(x->d2, psi, bytes), "hipMemcpy D2D (snapshot)");
+  lg_hip(hgrid_caxpy(x->d2, x->s1, d->prec, n, -1.0, 0.0), "caxpy");
+  lg_hip(hgrid_memcpy_d2d(x->s1, psi, bytes), "hipMemcpy D2D (snapshot)");
+  void *t = x->d1;
+  if (x->have == 1) {
+    x->d1 = x->d2;
+    x->d2 = t;
+    x->have = 2;
+    return 0;
+  }
+  const double n11 = creal(extrap_dot(g, x->d1, x->d1));
+  const double n22 = creal(extrap_dot(g, x->d2, x->d2));
+  const double complex x12 = extrap_dot(g, x->d1, x->d2);
+  const double r = creal(x12) / n11;
+  if (n11 > 0.0 && n22 > 0.0 && cabs(x12) > LG_EXTRAP_COS * sqrt(n11 * n22) && r > 0.05 && r < 0.995) {
+    *r_out = r;
+    lg_hip(hgrid_caxpy(psi, x->d2, d->prec, n, r / (1.0 - r), 0.0), "caxpy");
+    lg_c_written(g, 1);
+    grid_wf_normalize(gwf);
+    x->have = 0;
+    x->count++;
+    return 1;
+  }
+  x->d1 = x->d2;
+  x->d2 = t;
+  return 0;
+}
+
+static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy, INT stride,
+                      long *n_extrap) {
   grid_dev *d = gwf->grid->dev;
   const INT batch = 16;
 #if LG_RELAX_MIN_STRIDE % 16 != 0
 #error "the relaxation batches assume stopping-rule strides that are multiples of 16"
 #endif
+  if (stride < 0 || stride % batch) lg_fatal("grid_wf_gpe_relax_extrapolated: stride %ld is not a multiple of 16", (long)stride);
   double *hist = NULL;
   lg_ensure_device();
-  lg_hip(hgrid_malloc((void **)&hist, sizeof(double) * HGRID_RED_NCHAN * batch), "hipMalloc (energy history)");
+  lg_hip(hgrid_malloc((void **)&hist, sizeof(double) * HGRID_RED_NCHAN), "hipMalloc (energy history)");
   double host[HGRID_RED_NCHAN];
   double e = NAN;
-  INT done = -1, n = 0;
+  INT done = -1, n = 0, n0 = 0; /* n0: step at which the stopping rule (re)started */
   lg_relax_state rs;
   lg_relax_init(&rs);
+  extrap_state xs = {NULL, NULL, NULL, 0, 0};
   /* After the first batch every step is the same sequence of launches on the same buffers
    * (tables cached, scale pending, 3-D state deferred), so one batch of 16 steps is captured
-   * as a HIP graph and replayed: small grids are launch-bound otherwise. */
+   * as a HIP graph and replayed: small grids are launch-bound otherwise.  A snapshot or an
+   * extrapolation brings the state back to real space with no pending scale, so the batch
+   * after one runs eagerly and restores the captured form. */
   void *graph = NULL;
+  int eager = 1;
   while (n < max_steps && done < 0) {
     INT nb = (max_steps - n) < batch ? (max_steps - n) : batch;
-    if (graph && nb == batch) {
+    if (graph && nb == batch && !eager) {
       lg_hip(hgrid_graph_launch(graph), "hipGraphLaunch (relaxation)");
     } else {
-      const int capture = g_use_graphs && !graph && n > 0 && nb == batch;
+      const int capture = g_use_graphs && !graph && !eager && nb == batch;
       if (capture) lg_hip(hgrid_graph_begin(), "hipStreamBeginCapture (relaxation)");
       for (INT b = 0; b < nb; ++b) {
         /* the rule's checkpoints fall on multiples of its strides, all multiples of 16 from
-         * step 0, so only the first step of a batch computes its input state's energy */
+         * its start, so only the first step of a batch computes its input state's energy */
         gpe_step(gwf, vext, g_nl, -I * dt, 1, b == 0);
         if (b == 0)
           lg_hip(hgrid_memcpy_d2d(hist, d->dscal + GS_RED, sizeof(double) * HGRID_RED_NCHAN),
@@ -785,14 +861,45 @@ INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
         lg_hip(hgrid_graph_end(&graph), "hipStreamEndCapture (relaxation)");
         lg_hip(hgrid_graph_launch(graph), "hipGraphLaunch (relaxation)"); /* capture does not execute */
       }
+      eager = 0;
     }
     lg_hip(hgrid_memcpy_d2h(host, hist, sizeof(double) * HGRID_RED_NCHAN), "hipMemcpy D2H (energy history)");
     e = gpe_energy_from_parts(gwf, host, g_nl); /* the energy of the state entering step n */
-    if (lg_relax_feed(&rs, (long)n, e, tol)) done = n + 1;
+    if (lg_relax_feed(&rs, (long)(n - n0), e, tol)) done = n + 1;
     n += nb;
+    if (done < 0 && stride && nb == batch && n % stride == 0) {
+      double r = 0.0;
+      if (extrap_snapshot(gwf, &xs, &r)) {
+        lg_relax_init(&rs);
+        /* an overshoot leaves the energy rising towards the limit: in FP64 that is no rounding */
+        rs.rise_exit = d->prec != HGRID_C128;
+        /* what is left of the slowest mode decays by r per stride at most */
+        rs.lnq_floor = log(r) / (double)stride;
+        n0 = n;
+      }
+      eager = 1;
+    }
   }
   if (graph) lg_hip(hgrid_graph_free(graph), "hipGraphExecDestroy");
   lg_hip(hgrid_free(hist), "hipFree");
+  if (xs.s1) {
+    lg_hip(hgrid_free(xs.s1), "hipFree");
+    lg_hip(hgrid_free(xs.d1), "hipFree");
+    lg_hip(hgrid_free(xs.d2), "hipFree");
+  }
+  if (n_extrap) *n_extrap = xs.count;
   if (energy) *energy = e;
   return done < 0 ? n : done;
+}
+
+INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy) {
+  return relax_impl(gwf, vext, g_nl, dt, tol, max_steps, energy, 0, NULL);
+}
+
+INT grid_wf_gpe_relax_extrapolated(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy,
+                                   INT stride, INT *n_extrap) {
+  long ne = 0;
+  const INT steps = relax_impl(gwf, vext, g_nl, dt, tol, max_steps, energy, stride, &ne);
+  if (n_extrap) *n_extrap = ne;
+  return steps;
 }

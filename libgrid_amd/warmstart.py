@@ -61,9 +61,10 @@ class Problem:
         else:
             self.lg.grid_wf_warm_start(self.w, self.r, net, GAUSS_WIDTH)
 
-    def relax(self, net, fp32_tol=None, tol=1e-10, max_steps=200000, fp32_mode=b"\x01"):
+    def relax(self, net, fp32_tol=None, tol=1e-10, max_steps=200000, fp32_mode=b"\x01", stride=0):
         """Timed: start construction, optional FP32-storage relaxation to fp32_tol (fp32_mode 1:
-        FP32 arithmetic, 2: FP64 arithmetic), then FP64 to tol."""
+        FP32 arithmetic, 2: FP64 arithmetic), then FP64 to tol.  stride > 0 runs the FP64 stage
+        with vector extrapolation every stride steps (grid_wf_gpe_relax_extrapolated)."""
         lg = self.lg
         g = self.w.contents.grid
         e = C.c_double()
@@ -79,12 +80,17 @@ class Problem:
         if fp32_tol is not None:
             s32 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, fp32_tol, max_steps, C.byref(e))
             lg.cgrid_set_precision(g, b"\x00")
-        s64 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, tol, max_steps, C.byref(e))
+        nx = C.c_long(0)
+        if stride:
+            s64 = lg.grid_wf_gpe_relax_extrapolated(self.w, self.r, 0.0, DT, tol, max_steps, C.byref(e), stride,
+                                                    C.byref(nx))
+        else:
+            s64 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, tol, max_steps, C.byref(e))
         lg.grid_hip_sync()
         sec = time.perf_counter() - t0
         energy = lg.grid_wf_gpe_energy(self.w, self.r, 0.0)
         return dict(seconds=sec, fp32_steps=int(s32), fp64_steps=int(s64), energy=energy,
-                    converged=bool(s64 < max_steps))
+                    converged=bool(s64 < max_steps), extrapolations=int(nx.value))
 
 
 def potentials(seed, count, n):
@@ -153,9 +159,13 @@ def overlap(a, b):
     return float(np.sum(a * b) / np.sqrt(np.sum(a * a) * np.sum(b * b)))
 
 
-MODES = {  # name: (use the network, FP32-storage stage tolerance or None, its arithmetic)
+EXTRAP_STRIDE = 64
+MODES = {  # name: (use the network, FP32-storage stage tolerance or None, its arithmetic[, FP64 extrapolation stride])
     "gauss64": (False, None, None),
     "net64": (True, None, None),
+    # the FP64 stage with vector extrapolation (grid_wf_gpe_relax_extrapolated)
+    "gauss64e": (False, None, None, EXTRAP_STRIDE),
+    "net32e": (True, 3e-8, b"\x01", EXTRAP_STRIDE),
     # FP32 storage and arithmetic to 3e-8 Eh, then FP64.  The FP32 state's energy wanders by
     # about 1e-8 around the FP64 one, so a tighter FP32 stage mostly waits on noise.
     "gauss32": (False, 3e-8, b"\x01"),
@@ -179,9 +189,9 @@ def evaluate(lg, net, count, n=FINE, modes=("gauss64", "net32", "net64"), seed=H
         rec = dict(index=i, wells=len(prm[i]["depth"]))
         try:
             for m in modes:
-                use, t32, mode32 = MODES[m]
+                use, t32, mode32, *rest = MODES[m]
                 rec[m] = p.relax(net if use else None, fp32_tol=t32, tol=tol, max_steps=max_steps,
-                                 fp32_mode=mode32 or b"\x01")
+                                 fp32_mode=mode32 or b"\x01", stride=rest[0] if rest else 0)
         finally:
             p.free()
         es = [rec[m]["energy"] for m in modes]
@@ -200,7 +210,8 @@ def summarize(recs, modes):
         s[m] = dict(seconds_total=float(sum(r[m]["seconds"] for r in recs)),
                     fp32_steps_mean=float(np.mean([r[m]["fp32_steps"] for r in recs])),
                     fp64_steps_mean=float(np.mean([r[m]["fp64_steps"] for r in recs])),
-                    converged=int(sum(r[m]["converged"] for r in recs)))
+                    converged=int(sum(r[m]["converged"] for r in recs)),
+                    extrapolations_mean=float(np.mean([r[m].get("extrapolations", 0) for r in recs])))
     base = modes[0]
     for m in modes[1:]:
         ratios = [r[base]["seconds"] / r[m]["seconds"] for r in recs]

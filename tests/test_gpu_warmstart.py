@@ -69,3 +69,42 @@ def test_no_network_falls_back_to_gaussian(lg):
         assert abs(np.sum(np.abs(b) ** 2) * (WS.BOX / n) ** 3 - 1.0) < 1e-12
     finally:
         p.free()
+
+
+def test_extrapolated_relaxation_64(lg):
+    """grid_wf_gpe_relax_extrapolated against the plain relaxation from the Gaussian start on
+    three slow held-out 64^3 potentials: the same ground-state energy (both to 1e-10 Eh), fewer
+    steps in total, and the same result bit for bit with and without HIP-graph replay."""
+    case("extrapolated relaxation 64^3, held-out 0, 2, 15")
+    n = 64
+    prm = W.wells_3d_params(WS.HELDOUT_SEED, 16, WS.BOX)
+    e = C.c_double()
+    nx = C.c_long()
+    plain_steps = extra_steps = 0
+    for i in (0, 2, 15):
+        p = WS.Problem(lg, n, W.wells_3d(prm[i], n, WS.BOX / n))
+        try:
+            p.start(None)
+            sp = lg.grid_wf_gpe_relax(p.w, p.r, 0.0, WS.DT, 1e-10, 200000, C.byref(e))
+            ep = lg.grid_wf_gpe_energy(p.w, p.r, 0.0)
+            runs = []
+            for graphs in (1, 0):
+                lg.grid_hip_graphs(graphs)
+                p.start(None)
+                sx = lg.grid_wf_gpe_relax_extrapolated(p.w, p.r, 0.0, WS.DT, 1e-10, 200000, C.byref(e), 64,
+                                                       C.byref(nx))
+                runs.append((sx, nx.value, LG.get_cgrid(p.w.contents.grid)))
+            lg.grid_hip_graphs(1)
+            ex = lg.grid_wf_gpe_energy(p.w, p.r, 0.0)
+            print(f"potential {i}: plain {sp} steps E {ep:.13f}; extrapolated {runs[0][0]} steps "
+                  f"({runs[0][1]} extrapolations) E {ex:.13f}")
+            assert sp < 200000 and runs[0][0] < 200000
+            assert abs(ex - ep) < 2e-10
+            assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+            np.testing.assert_array_equal(runs[0][2], runs[1][2])
+            plain_steps += sp
+            extra_steps += runs[0][0]
+        finally:
+            lg.grid_hip_graphs(1)
+            p.free()
+    assert extra_steps < 0.7 * plain_steps

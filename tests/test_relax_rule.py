@@ -14,13 +14,24 @@ HDR = os.path.join(HERE, "..", "libgrid_amd", "csrc", "host")
 
 SRC = r"""
 #include "relax_rule.h"
-long run(const double *e, long n, double tol) {
+long run_floor(const double *e, long n, double tol, int rise_exit, double lnq_floor) {
   lg_relax_state s;
   lg_relax_init(&s);
+  s.rise_exit = rise_exit;
+  s.lnq_floor = lnq_floor;
   for (long k = 0; k < n; ++k)
     if (lg_relax_feed(&s, k, e[k], tol)) return k;
   return -1;
 }
+long run_ex(const double *e, long n, double tol, int rise_exit) {
+  lg_relax_state s;
+  lg_relax_init(&s);
+  s.rise_exit = rise_exit;
+  for (long k = 0; k < n; ++k)
+    if (lg_relax_feed(&s, k, e[k], tol)) return k;
+  return -1;
+}
+long run(const double *e, long n, double tol) { return run_ex(e, n, tol, 1); }
 """
 
 
@@ -32,12 +43,15 @@ def rule(tmp_path_factory):
     so = d / "r.so"
     subprocess.check_call(["gcc", "-O2", "-shared", "-fPIC", "-I", HDR, str(c), "-o", str(so), "-lm"])
     lib = C.CDLL(str(so))
-    lib.run.restype = C.c_long
-    lib.run.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_double]
+    lib.run_ex.restype = C.c_long
+    lib.run_ex.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_double, C.c_int]
 
-    def stop(e, tol):
+    lib.run_floor.restype = C.c_long
+    lib.run_floor.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_double, C.c_int, C.c_double]
+
+    def stop(e, tol, rise_exit=1, lnq_floor=0.0):
         a = np.ascontiguousarray(e, dtype=np.float64)
-        return lib.run(a.ctypes.data_as(C.POINTER(C.c_double)), len(a), tol)
+        return lib.run_floor(a.ctypes.data_as(C.POINTER(C.c_double)), len(a), tol, rise_exit, lnq_floor)
 
     return stop
 
@@ -101,3 +115,37 @@ def test_fast_transient_on_slow_tail(rule):
         rem = e[n] + 0.2
         print(f"a {a:.0e}: stopped at {n}, remaining {rem:.2e}")
         assert rem < 3 * tol
+
+
+@pytest.mark.parametrize("rho", [0.97, 0.995, 0.9996])
+def test_tail_from_below_after_extrapolation(rule, rho):
+    """An extrapolation that overshoots a slow mode leaves E_n = E_inf - a rho^n, a rise.  With
+    rise_exit = 0 the rule treats it as a geometric tail and stops only once |E_n - E_inf| is
+    below tol; the default rule (rise_exit = 1) would take the first small rise for rounding
+    and stop with the error still 3e-10."""
+    tol = 1e-10
+    k = np.arange(400000)
+    einf = -0.08
+    e = einf - 3e-10 * rho ** k
+    assert rule(e, tol, 1) == 32  # the default: stops at the third checkpoint
+    s = rule(e, tol, 0)
+    assert s > 0 and abs(e[s] - einf) < tol
+    # the same for a tail from above, unchanged by the flag
+    e2 = einf + 3e-10 * rho ** k
+    s2 = rule(e2, tol, 0)
+    assert s2 == rule(e2, tol, 1) and e2[s2] - einf < tol
+
+
+def test_rate_floor_after_extrapolation(rule):
+    """After an extrapolation a fast component it excited (1e-7, rho 0.95) sits on what is left
+    of the slow mode (3e-10, rho 0.9994).  Without a floor the ratios of 16-step windows are the
+    fast one's and the rule stops with 3e-10 still to come; with the slow rate as the floor it
+    waits until the remainder is below tol."""
+    tol = 1e-10
+    k = np.arange(200000)
+    einf = -0.08
+    e = einf + 1e-7 * 0.95 ** k + 3e-10 * 0.9994 ** k
+    early = rule(e, tol, 0)
+    assert e[early] - einf > 2e-10
+    s = rule(e, tol, 0, np.log(0.9994))
+    assert e[s] - einf < tol
