@@ -142,10 +142,11 @@ def profiled_traffic(pass_name, n):
 def ground_state(lib, LG, psi, vext, prm, n, max_steps):
     """Wall time to |E_n - E_{n-1}| < 1e-10 Eh from the workload's initial state:
     (a) FP64 throughout; (b) FP32 storage until the energy change falls below 1e-7 Eh (the
-    FP32 noise floor is ~1e-8), then FP64 to 1e-10.  Energies of both runs are reported."""
+    FP32 noise floor is ~1e-8), then FP64 to 1e-10; (c) as (b) with vector extrapolation every
+    64 steps in the FP64 stage.  Energies of all runs are reported."""
     step, g, dt, tol = prm["step"], prm["g"], prm["dt"], prm["tol"]
     out = {}
-    for mode in ("fp64", "fp32_then_fp64"):
+    for mode in ("fp64", "fp32_then_fp64", "fp32_then_fp64e"):
         w = lib.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"gs")
         vr = lib.rgrid_alloc(n, n, n, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vext")
         LG.set_cgrid(w.contents.grid, psi)
@@ -153,7 +154,7 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
         e = C.c_double()
         # upload and table set-up outside the timed region
         lib.grid_wf_gpe_energy(w, vr, g)
-        if mode == "fp32_then_fp64":
+        if mode != "fp64":
             lib.cgrid_set_precision(w.contents.grid, b"\x01")
         lib.grid_hip_sync()
         t0 = time.perf_counter()
@@ -163,7 +164,10 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
         else:
             s32 = lib.grid_wf_gpe_relax(w, vr, g, dt, 1e-7, max_steps, C.byref(e))
             lib.cgrid_set_precision(w.contents.grid, b"\x00")
-            steps = lib.grid_wf_gpe_relax(w, vr, g, dt, tol, max_steps, C.byref(e))
+            if mode == "fp32_then_fp64e":
+                steps = lib.grid_wf_gpe_relax_extrapolated(w, vr, g, dt, tol, max_steps, C.byref(e), 64, None)
+            else:
+                steps = lib.grid_wf_gpe_relax(w, vr, g, dt, tol, max_steps, C.byref(e))
         lib.grid_hip_sync()
         el = time.perf_counter() - t0
         efinal = lib.grid_wf_gpe_energy(w, vr, g)
@@ -172,6 +176,7 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
         lib.grid_wf_free(w)
         lib.rgrid_free(vr)
     out["speedup_fp32_then_fp64"] = out["fp64"]["seconds"] / out["fp32_then_fp64"]["seconds"]
+    out["speedup_fp32_then_fp64e"] = out["fp64"]["seconds"] / out["fp32_then_fp64e"]["seconds"]
     out["tol_Eh"] = tol
     return out
 
@@ -179,14 +184,16 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
 def warm_start_bench(lib, count):
     """Wall time to a 1e-10 Eh ground state on held-out 256^3 wells potentials: plain Gaussian
     start with FP64 throughout, against the trained network's start with FP32 relaxation
-    (to 3e-8) then FP64 (to 1e-10).  Both timings include building the start state."""
+    (to 3e-8) then FP64 (to 1e-10), with (net32e) and without (net32) vector extrapolation in
+    the FP64 stage, and against the network start in FP64 alone (net64).  All timings include
+    building the start state."""
     from libgrid_amd import warmstart as WS
 
     if not os.path.exists(WS.WEIGHTS):
         log(f"warm start: no trained network at {WS.WEIGHTS}; skipped")
         return None
     net = lib.grid_surrogate_load(WS.WEIGHTS.encode(), 1)
-    modes = ("gauss64", "net32", "net64")
+    modes = ("gauss64", "net32e", "net32", "net64")
     recs = WS.evaluate(lib, net, count, WS.FINE, modes)
     lib.grid_surrogate_free(net)
     summ = WS.summarize(recs, modes)
