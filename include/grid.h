@@ -298,6 +298,9 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
  * no cap) are continued one at a time on the multi-kernel FP32 path, which spreads one grid over
  * the whole GPU; grid_batch2d_last_phase2 reports how many and their wall time. */
 void grid_batch2d_set_straggler_cap(INT steps);
+/* The straggler phase relaxes with vector extrapolation every stride steps
+ * (grid_wf_gpe_relax_extrapolated; default 64); 0 relaxes the stragglers plainly. */
+void grid_batch2d_set_straggler_extrapolation(INT stride);
 void grid_batch2d_last_phase2(INT *count, REAL *seconds);
 /* Time every tile configuration of the batched kernel on these problems for probe_steps steps,
  * keep the fastest (also cached in $XDG_CACHE_HOME or ~/.cache/libgrid_amd/batch2d_tile.txt)

@@ -21,6 +21,7 @@ static const b2d_tile kCandidates[] = {{0, 256, 1}, {0, 256, 2}, {0, 512, 1}, {1
 
 static b2d_tile g_tile = {-1, 0, 0};
 static INT g_cap = 3000;            /* persistent-launch step cap (0: none) */
+static INT g_phase2_stride = 64;    /* vector-extrapolation stride of the straggler phase (0: plain) */
 static INT g_phase2_count = 0;      /* solves continued on the multi-kernel path by the last call */
 static double g_phase2_seconds = 0.0;
 static char g_report[2048];
@@ -366,7 +367,10 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
     const char prec0 = cgrid_get_precision(g);
     cgrid_set_precision(g, GRID_PREC_SINGLE);
     REAL e2 = 0.0;
-    const INT s2 = grid_wf_gpe_relax(gwf[i], potential[i], 0.0, dt, tol, max_steps - cap, &e2);
+    const INT s2 = g_phase2_stride
+                       ? grid_wf_gpe_relax_extrapolated(gwf[i], potential[i], 0.0, dt, tol, max_steps - cap, &e2,
+                                                        g_phase2_stride, NULL)
+                       : grid_wf_gpe_relax(gwf[i], potential[i], 0.0, dt, tol, max_steps - cap, &e2);
     cgrid_set_precision(g, prec0);
     if (energy) energy[i] = e2;
     if (steps) steps[i] = cap + s2;
@@ -379,6 +383,11 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
 }
 
 void grid_batch2d_set_straggler_cap(INT steps) { g_cap = steps; }
+
+void grid_batch2d_set_straggler_extrapolation(INT stride) {
+  if (stride < 0 || stride % 16) lg_fatal("grid_batch2d_set_straggler_extrapolation: stride %ld is not a multiple of 16", (long)stride);
+  g_phase2_stride = stride;
+}
 
 void grid_batch2d_last_phase2(INT *count, REAL *seconds) {
   if (count) *count = g_phase2_count;
