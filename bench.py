@@ -236,6 +236,48 @@ def fp32_512_bench(lib, LG, steps=20, warmup=3):
     return out
 
 
+def propagators_bench(lib, LG, steps=20, warmup=3):
+    """grid_wf_propagate at 256^3 FP64 (spacing 0.25, real time dt 0.05, complex cgrid
+    potential of the barrier-trap workload) for each FFT propagator: the plain 2nd-order step,
+    its k-space cutoff (CFFT, kmax at half the Nyquist |k|), the 4th-order step and the
+    absorbing boundary (layers of 32 points on every face).  Model bytes of the 2nd-order step:
+    400 B/pt."""
+    n, step, dt = 256, 0.25, 0.05
+    x = (np.arange(n) - n // 2) * step
+    r2 = (x * x)[:, None, None] + (x * x)[None, :, None] + (x * x)[None, None, :]
+    pot = (0.5 * 0.01 * r2 + 0.05 * np.exp(-r2 / 4.0)).astype(np.complex128)
+    del r2
+    vp = lib.cgrid_alloc(n, n, n, step, LG.boundary("cgrid_value_outside_periodic"), None, b"pot")
+    LG.set_cgrid(vp, pot)
+    del pot
+    kmax = 0.5 * np.sqrt(3.0) * np.pi / step
+    cases = (("fft2", LG.WF_2ND_ORDER_FFT, None), ("cfft2", LG.WF_2ND_ORDER_CFFT, None),
+             ("fft4", LG.WF_4TH_ORDER_FFT, None), ("fft2_absorbing", LG.WF_2ND_ORDER_FFT, (32, 223) * 3))
+    out = {}
+    for name, prop, bounds in cases:
+        w = lib.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, prop, b"wf")
+        w.contents.kmax = kmax
+        lib.grid_wf_gaussian_start(w, 4.0)
+        if bounds:
+            lib.grid_wf_boundary(w, None, *bounds)
+        for _ in range(warmup):
+            lib.grid_wf_propagate(w, vp, LG.c_complex.of(dt))
+        lib.grid_hip_sync()
+        t = LG.HipTimer()
+        t.start()
+        for _ in range(steps):
+            lib.grid_wf_propagate(w, vp, LG.c_complex.of(dt))
+        ms = t.stop() / steps
+        t.close()
+        lib.grid_wf_free(w)
+        out[name] = dict(ms_per_step=ms, steps_per_s=1000.0 / ms,
+                         model_frac=400 * n ** 3 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS)
+        log(f"propagator {name:15s} {ms:.3f} ms/step ({1000 / ms:.0f} steps/s)")
+    lib.cgrid_free(vp)
+    out["workload"] = "grid_wf_propagate 256^3 complex128, real time dt 0.05, complex cgrid potential"
+    return out
+
+
 def batch2d_bench(lib, LG, count=256, tol=1e-8, dt=0.1):
     """The batched 2-D workload: 256 seeded problems (n in 64..256, 3-8 Gaussian wells, spacing
     0.2) relaxed to tol in one persistent launch (autotuned tile)."""
@@ -362,7 +404,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gs", action="store_true", help="skip the time-to-ground-state runs")
     ap.add_argument("--gs-max-steps", type=int, default=20000)
-    ap.add_argument("--no-rows", action="store_true", help="skip the FP32 512^3 and batched 2-D measurements")
+    ap.add_argument("--no-rows", action="store_true", help="skip the FP32 512^3, batched 2-D and propagator-variant measurements")
     ap.add_argument("--ws-count", type=int, default=2,
                     help="held-out 256^3 wells potentials for the warm-start time-to-ground-state comparison")
     args = ap.parse_args()
@@ -430,6 +472,7 @@ def main():
     if not args.no_rows:
         rows["fp32_512"] = fp32_512_bench(lib, LG)
         rows["batch2d"] = batch2d_bench(lib, LG)
+        rows["propagators"] = propagators_bench(lib, LG)
 
     cpu = None
     if not args.no_cpu:
