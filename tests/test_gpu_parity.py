@@ -307,3 +307,29 @@ def test_single_storage_fp64_arithmetic_path(lg, orc, steps):
     assert drift < 2e-6 * np.sqrt(steps / 20)
     lg.grid_wf_free(w)
     lg.cgrid_free(vp)
+
+
+def test_add_scaled_and_merge(lg):
+    """cgrid_add_scaled (reference src/cgrid.c:1303-1319) against numpy, with a pending scale on
+    the source, and grid_wf_merge (src/grid_wf.c:728-755) against its definition with
+    grid_wf_absorb's amplitude."""
+    case("cgrid_add_scaled and grid_wf_merge, 16x12x20")
+    shape = (16, 12, 20)
+    a, c = rand_c(shape, 31), rand_c(shape, 32)
+    ga, gc = new_cgrid(lg, a), new_cgrid(lg, c)
+    lg.cgrid_multiply(ga, LG.c_complex.of(0.5 - 0.25j))  # pending scale, folded in by the call
+    d = 1.5 + 0.75j
+    lg.cgrid_add_scaled(gc, LG.c_complex.of(d), ga)
+    assert rel_l2(LG.get_cgrid(gc), c + d * (0.5 - 0.25j) * a) < 1e-15
+    b = (3, 11, 2, 8, 4, 14)
+    w = new_wf(lg, c, 0.2)
+    lg.grid_wf_boundary(w, None, *b)
+    gr, gi = new_cgrid(lg, a), new_cgrid(lg, c)
+    lg.grid_wf_merge(w, gr, gi)
+    alpha = np.array([[[lg.grid_wf_absorb(i, j, k, *b) for k in range(shape[2])] for j in range(shape[1])]
+                      for i in range(shape[0])])
+    assert np.any(alpha == 0) and np.any(alpha == 1) and np.any((alpha > 0) & (alpha < 1))
+    assert rel_l2(LG.get_cgrid(w.contents.grid), (1 - alpha) * a + alpha * c) < 1e-15
+    for g in (ga, gc, gr, gi):
+        lg.cgrid_free(g)
+    lg.grid_wf_free(w)
