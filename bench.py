@@ -184,16 +184,16 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
 def warm_start_bench(lib, count):
     """Wall time to a 1e-10 Eh ground state on held-out 256^3 wells potentials: plain Gaussian
     start with FP64 throughout, against the trained network's start with FP32 relaxation
-    (to 3e-8) then FP64 (to 1e-10), with (net32e) and without (net32) vector extrapolation in
-    the FP64 stage, and against the network start in FP64 alone (net64).  All timings include
-    building the start state."""
+    (to 3e-8) then FP64 (to 1e-10): with vector extrapolation in both stages (net32ee, the
+    fast path), in the FP64 stage only (net32e) and in neither (net32), and against the network
+    start in FP64 alone (net64).  All timings include building the start state."""
     from libgrid_amd import warmstart as WS
 
     if not os.path.exists(WS.WEIGHTS):
         log(f"warm start: no trained network at {WS.WEIGHTS}; skipped")
         return None
     net = lib.grid_surrogate_load(WS.WEIGHTS.encode(), 1)
-    modes = ("gauss64", "net32e", "net32", "net64")
+    modes = ("gauss64", "net32ee", "net32e", "net32", "net64")
     recs = WS.evaluate(lib, net, count, WS.FINE, modes)
     lib.grid_surrogate_free(net)
     summ = WS.summarize(recs, modes)

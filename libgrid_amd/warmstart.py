@@ -61,10 +61,11 @@ class Problem:
         else:
             self.lg.grid_wf_warm_start(self.w, self.r, net, GAUSS_WIDTH)
 
-    def relax(self, net, fp32_tol=None, tol=1e-10, max_steps=200000, fp32_mode=b"\x01", stride=0):
+    def relax(self, net, fp32_tol=None, tol=1e-10, max_steps=200000, fp32_mode=b"\x01", stride=0, stride32=0):
         """Timed: start construction, optional FP32-storage relaxation to fp32_tol (fp32_mode 1:
         FP32 arithmetic, 2: FP64 arithmetic), then FP64 to tol.  stride > 0 runs the FP64 stage
-        with vector extrapolation every stride steps (grid_wf_gpe_relax_extrapolated)."""
+        and stride32 > 0 the FP32 stage with vector extrapolation every that many steps
+        (grid_wf_gpe_relax_extrapolated)."""
         lg = self.lg
         g = self.w.contents.grid
         e = C.c_double()
@@ -77,10 +78,15 @@ class Problem:
         if fp32_tol is not None:
             lg.cgrid_set_precision(g, fp32_mode)
         self.start(net)
-        if fp32_tol is not None:
-            s32 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, fp32_tol, max_steps, C.byref(e))
-            lg.cgrid_set_precision(g, b"\x00")
         nx = C.c_long(0)
+        n32 = C.c_long(0)
+        if fp32_tol is not None:
+            if stride32:
+                s32 = lg.grid_wf_gpe_relax_extrapolated(self.w, self.r, 0.0, DT, fp32_tol, max_steps, C.byref(e),
+                                                        stride32, C.byref(n32))
+            else:
+                s32 = lg.grid_wf_gpe_relax(self.w, self.r, 0.0, DT, fp32_tol, max_steps, C.byref(e))
+            lg.cgrid_set_precision(g, b"\x00")
         if stride:
             s64 = lg.grid_wf_gpe_relax_extrapolated(self.w, self.r, 0.0, DT, tol, max_steps, C.byref(e), stride,
                                                     C.byref(nx))
@@ -90,7 +96,7 @@ class Problem:
         sec = time.perf_counter() - t0
         energy = lg.grid_wf_gpe_energy(self.w, self.r, 0.0)
         return dict(seconds=sec, fp32_steps=int(s32), fp64_steps=int(s64), energy=energy,
-                    converged=bool(s64 < max_steps), extrapolations=int(nx.value))
+                    converged=bool(s64 < max_steps), extrapolations=int(nx.value + n32.value))
 
 
 def potentials(seed, count, n):
@@ -166,6 +172,7 @@ MODES = {  # name: (use the network, FP32-storage stage tolerance or None, its a
     # the FP64 stage with vector extrapolation (grid_wf_gpe_relax_extrapolated)
     "gauss64e": (False, None, None, EXTRAP_STRIDE),
     "net32e": (True, 3e-8, b"\x01", EXTRAP_STRIDE),
+    "net32ee": (True, 3e-8, b"\x01", EXTRAP_STRIDE, EXTRAP_STRIDE),  # both stages extrapolated
     # FP32 storage and arithmetic to 3e-8 Eh, then FP64.  The FP32 state's energy wanders by
     # about 1e-8 around the FP64 one, so a tighter FP32 stage mostly waits on noise.
     "gauss32": (False, 3e-8, b"\x01"),
@@ -191,7 +198,8 @@ def evaluate(lg, net, count, n=FINE, modes=("gauss64", "net32", "net64"), seed=H
             for m in modes:
                 use, t32, mode32, *rest = MODES[m]
                 rec[m] = p.relax(net if use else None, fp32_tol=t32, tol=tol, max_steps=max_steps,
-                                 fp32_mode=mode32 or b"\x01", stride=rest[0] if rest else 0)
+                                 fp32_mode=mode32 or b"\x01", stride=rest[0] if rest else 0,
+                                 stride32=rest[1] if len(rest) > 1 else 0)
         finally:
             p.free()
         es = [rec[m]["energy"] for m in modes]
