@@ -14,7 +14,12 @@
  * against constant / linear / absorbing potentials; norm and energies against analytic
  * Gaussian integrals; the full step against Ehrenfest dynamics, time reversibility and the
  * harmonic-oscillator ground state; the GPE composition against the same invariants with
- * g = 0 and against a first-order expansion in g.
+ * g = 0 and against a first-order expansion in g; the cut-off (CFFT) kinetic phase against its
+ * limits, the Nyquist convention of its test and plane-wave eigenstates; the 4th-order step
+ * against the 2nd-order one without a potential, constant potentials, reversibility and the
+ * closed-form imaginary-time fixed point of its stage sequence in a harmonic trap; the
+ * absorbing boundary against hand-computed amplitudes, its alpha = 0 / alpha = 1 regions and
+ * the absorption of an outgoing packet.
  *
  * Reductions: the reference sums with OpenMP reduction(+) in thread order; the oracle sums
  * each [nx][ny] row in index order and the row sums with Neumaier compensation, so results
