@@ -294,7 +294,7 @@ void grid_hip_graphs(int enable);
  * device runs grid_batch2d_autotune unless a tuned tile is cached. */
 REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, REAL tol, INT max_steps, REAL *energy,
                            INT *steps);
-/* Solves still unconverged after this many steps of the persistent launch (default 3000; 0 =
+/* Solves still unconverged after this many steps of the persistent launch (default 2000; 0 =
  * no cap) are continued one at a time on the multi-kernel FP32 path, which spreads one grid over
  * the whole GPU; grid_batch2d_last_phase2 reports how many and their wall time. */
 void grid_batch2d_set_straggler_cap(INT steps);

@@ -20,7 +20,7 @@ static const b2d_tile kCandidates[] = {{0, 256, 1}, {0, 256, 2}, {0, 512, 1}, {1
 #define NCAND ((int)(sizeof(kCandidates) / sizeof(kCandidates[0])))
 
 static b2d_tile g_tile = {-1, 0, 0};
-static INT g_cap = 3000;            /* persistent-launch step cap (0: none) */
+static INT g_cap = 2000;            /* persistent-launch step cap (0: none) */
 static INT g_phase2_stride = 64;    /* vector-extrapolation stride of the straggler phase (0: plain) */
 static INT g_phase2_count = 0;      /* solves continued on the multi-kernel path by the last call */
 static double g_phase2_seconds = 0.0;
