@@ -86,6 +86,10 @@ int hgrid_runpack(double *dst, const double *src, long rows, long nz, long nz2);
                                         applied as an FP64 real scale */
 #define HGRID_PF_KCUT       0x1000u  /* with MID_KIN: zero psi where (kc_a[n] + kc_1[c1]) + kc_2[c2] >= kcut2
                                         (the k-space cutoff of the CFFT propagator) */
+#define HGRID_PF_POT_NOFFT1 0x2000u  /* with MID_POT: no transform before the potential factor
+                                        (factor, then transform dir2); nfft = 1 */
+#define HGRID_PF_POT_NOFFT2 0x4000u  /* with MID_POT: no transform after it (transform dir1, then
+                                        factor); nfft = 1.  MID_POT passes also take PRESCALE */
 
 /* Reduction channels (partials[chan * nblocks + block]). */
 #define HGRID_RED_NORM   0   /* sum |psi|^2 of the pass input   (PRE_RED) */

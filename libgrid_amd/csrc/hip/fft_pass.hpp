@@ -216,7 +216,7 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
 
   double acc[HGRID_RED_NCHAN] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
-  if constexpr (KIND == KIND_FFT) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_POT) {
     if (fl & HGRID_PF_PRESCALE) {
       const C s = mk<C>((Rl)a.scale[0], (Rl)a.scale[1]);
 #pragma unroll
@@ -250,7 +250,12 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     if ((fl & PRE_ANY) && !(fl & HGRID_PF_PRE_AFTER)) pre_ops();
   }
 
-  fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
+  if constexpr (KIND == KIND_POT) {
+    // the 4th-order step's outer V stages: potential factor before or after one transform
+    if (!(fl & HGRID_PF_POT_NOFFT1)) fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
+  } else {
+    fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
+  }
 
   if constexpr (KIND == KIND_FFT) {
     if ((fl & PRE_ANY) && (fl & HGRID_PF_PRE_AFTER)) pre_ops();
@@ -332,7 +337,11 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     }
   }
 
-  if constexpr (KIND != KIND_FFT) fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
+  if constexpr (KIND == KIND_POT) {
+    if (!(fl & HGRID_PF_POT_NOFFT2)) fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
+  } else if constexpr (KIND != KIND_FFT) {
+    fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
+  }
 
   if constexpr (KIND == KIND_FFT) {
     if (a.post != 1.0) {

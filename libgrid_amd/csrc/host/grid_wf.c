@@ -289,6 +289,9 @@ typedef struct {
   double c_re, c_im, cons;
   double tm_re, tm_im;   /* time_mass of the half step */
   int no_defer;          /* leave the result in real space (no z transform kept pending) */
+  const void *pot6;      /* 4th order: exp(c6 (cons + pot6)) applied before the first and after
+                          * the last transform of the step (its outer V(t/6) stages) */
+  double c6_re, c6_im;
   int kcut;              /* CFFT: zero the components with |k - k0|^2 >= kcut2 */
   double kcut2;
   int want_energy;       /* reductions for E of the input state */
@@ -340,6 +343,16 @@ static void fused_step(wf *gwf, const step_args *sa) {
     p.g = sa->g;
   }
   if (sa->want_energy) { p.flags |= HGRID_PF_PRE_RED; p.vext = sa->vext; p.partials = part1; }
+  if (sa->pot6) {
+    /* V(t/6), then the forward transform (never with zin: the 4th-order step is not deferred) */
+    p.flags |= HGRID_PF_MID_POT | HGRID_PF_POT_NOFFT1;
+    p.dir2 = -1;
+    p.pot = sa->pot6;
+    p.c_re = sa->c6_re;
+    p.c_im = sa->c6_im;
+    p.cons = sa->cons;
+    p.mid = 1.0;
+  }
   if (zin) {
     p.dir1 = +1;
     p.flags |= HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE;
@@ -444,6 +457,15 @@ static void fused_step(wf *gwf, const step_args *sa) {
     lg_pass_init(&p, &s, prec, psi, psi, 2);
     p.dir1 = +1;
     if (sa->want_post_norm) { p.flags |= HGRID_PF_POST_RED; p.partials = part9; }
+    if (sa->pot6) {
+      /* the inverse transform, then V(t/6) */
+      p.flags |= HGRID_PF_MID_POT | HGRID_PF_POT_NOFFT2;
+      p.pot = sa->pot6;
+      p.c_re = sa->c6_re;
+      p.c_im = sa->c6_im;
+      p.cons = sa->cons;
+      p.mid = 1.0;
+    }
     lg_run_pass(&p);
   } else {
     nbpost = nb1;
@@ -563,13 +585,18 @@ static void propagate_split(wf *gwf, cgrid *potential, REAL complex time, int re
     if (potential->dev->prec != g->dev->prec) cgrid_set_precision(potential, cgrid_get_precision(g));
     sa.pot = lg_c_in_flushed(potential);
   }
-  /* 4th order (reference src/grid_wf.c:433-457): the outer V(t/6) stages run as pointwise
-   * sweeps around a fused K(t/2) V(2t/3) K(t/2).  The reference also forms
+  /* 4th order (reference src/grid_wf.c:433-457): the outer V(t/6) stages ride on the first
+   * and last z passes of a fused K(t/2) V(2t/3) K(t/2).  The reference also forms
    * (1/48)(hbar^2/m)|t|^2 |grad V|^2 in a workspace (:449-451) but hands the unmodified
    * potential to the V(2t/3) stage (:452); that term never reaches psi, so it is not formed. */
   const int fourth = fourth_order(gwf);
   const REAL complex tv = fourth ? 2.0 * time / 3.0 : time;
-  if (fourth) grid_wf_propagate_potential(gwf, time / 6.0, potential, cons);
+  if (fourth && sa.pot) {
+    const REAL complex t6 = time / 6.0;
+    sa.pot6 = sa.pot;
+    sa.c6_re = cimag(t6) / HBAR;
+    sa.c6_im = -creal(t6) / HBAR;
+  }
   sa.c_re = cimag(tv) / HBAR;
   sa.c_im = -creal(tv) / HBAR;
   sa.cons = cons;
@@ -578,9 +605,8 @@ static void propagate_split(wf *gwf, cgrid *potential, REAL complex time, int re
   sa.tm_im = cimag(tm);
   sa.kcut = uses_cutoff(gwf);
   sa.kcut2 = gwf->kmax * gwf->kmax;
-  sa.no_defer = fourth || real_space; /* the V(t/6) stage and the merge read the real-space state */
+  sa.no_defer = fourth || real_space; /* the last V(t/6) stage and the merge act in real space */
   fused_step(gwf, &sa);
-  if (fourth) grid_wf_propagate_potential(gwf, time / 6.0, potential, cons);
 }
 
 /* ---- energies ----------------------------------------------------------------------------- */
