@@ -248,7 +248,7 @@ def main(argv=None):
     e.add_argument("--count", type=int, default=200)
     e.add_argument("--first", type=int, default=0)
     e.add_argument("--n", type=int, default=FINE)
-    e.add_argument("--modes", default="gauss64,net32,net64,gauss32")
+    e.add_argument("--modes", default="gauss64,net32ee,net64")
     e.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     lg = load()
