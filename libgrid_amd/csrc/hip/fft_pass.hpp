@@ -305,11 +305,19 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
     const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
     const auto rrho = make_rsrc(a.rho, a.rho ? (unsigned long)ntot * 8 : 0);
     if constexpr (KIND == KIND_POT) {
+      // loads first, then the factors in groups of four behind scheduling fences, so that few
+      // exp / sincos evaluations are live at once (234 -> 178 VGPRs at L = 256 in FP64)
+      S pv[R];
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const S pv = bload<S>(rpot, (e0 + (unsigned)j * estep) * CS, 0);
-        v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, a.cons + (double)pv.x, (double)pv.y));
+      for (int j = 0; j < R; ++j) pv[j] = bload<S>(rpot, (e0 + (unsigned)j * estep) * CS, 0);
+#pragma unroll
+      for (int j0 = 0; j0 < R; j0 += 4) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = j0; j < (j0 + 4 < R ? j0 + 4 : R); ++j)
+          v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, a.cons + (double)pv[j].x, (double)pv[j].y));
       }
+      __builtin_amdgcn_sched_barrier(0);
     } else {
       // real potential: V = vext + g rho, rho alone when it already holds V_eff, or vext alone
       double vr[R];
