@@ -13,7 +13,12 @@ so each oracle function is checked against something it does not contain:
     potential = decay, Ehrenfest dynamics in a harmonic trap, time reversibility, norm
     conservation, the harmonic-oscillator ground state E0 = 3/2 hbar omega;
   * norm and energies: analytic Gaussian integrals; FD energy converges as O(h^2);
-  * GPE composition: uniform condensates (closed form) and the g = 0 limit.
+  * GPE composition: uniform condensates (closed form) and the g = 0 limit;
+  * the 4th-order step: equal to the 2nd-order step without a potential, a constant potential
+    as a global phase, reversibility, and the closed-form imaginary-time fixed point of its
+    stage sequence in a harmonic trap (which also pins the 2nd-order sequence);
+  * the absorbing boundary: hand-computed amplitudes, the real-time / imaginary-time regions
+    of the merged step, and the absorption of an outgoing packet.
 """
 import os
 
