@@ -127,3 +127,45 @@ def test_every_struct_mirror_matches_c_field_by_field():
         py = mirrors[cname]
         want = C.sizeof(py) if field == "sizeof" else getattr(py, field).offset
         assert int(val) == want, (cname, field, int(val), want)
+
+
+def test_wf_boundary_fields():
+    """grid_wf_boundary (reference src/grid_wf.c:99-120) is host bookkeeping: it sets the box and
+    ts_func on the wavefunction and ts_func on the predict wavefunction; all bounds 0 clears
+    them, and of the predict wavefunction's bounds only lx, as the reference does."""
+    lib = libgrid_amd.load()
+    w = lib.grid_wf_alloc(4, 4, 4, 0.1, 1.0, libgrid_amd.WF_PERIODIC_BOUNDARY, libgrid_amd.WF_2ND_ORDER_FFT, b"w")
+    p = lib.grid_wf_alloc(4, 4, 4, 0.1, 1.0, libgrid_amd.WF_PERIODIC_BOUNDARY, libgrid_amd.WF_2ND_ORDER_FFT, b"p")
+    try:
+        p.contents.ly = 7
+        lib.grid_wf_boundary(w, p, 1, 2, 3, 4, 5, 6)
+        c = w.contents
+        assert (c.lx, c.hx, c.ly, c.hy, c.lz, c.hz) == (1, 2, 3, 4, 5, 6)
+        assert c.ts_func and p.contents.ts_func and p.contents.lx == 0 and p.contents.ly == 7
+        p.contents.lx = 9
+        lib.grid_wf_boundary(w, p, 0, 0, 0, 0, 0, 0)
+        assert (c.lx, c.hx, c.ly, c.hy, c.lz, c.hz) == (0,) * 6 and not c.ts_func
+        assert not p.contents.ts_func and p.contents.lx == 0 and p.contents.ly == 7
+    finally:
+        lib.grid_wf_free(w)
+        lib.grid_wf_free(p)
+
+
+@pytest.mark.parametrize("propagator,setup,message", [
+    (5, "", "Propagator 5 is not implemented"),
+    (2, "lib.grid_wf_boundary(w, None, 1, 2, 1, 2, 1, 2);", "4th order propagator not available for FFT absorbing BC"),
+])
+def test_unsupported_propagator_combinations_abort(propagator, setup, message):
+    """Propagators this build does not implement, and the 4th-order step with an absorbing
+    boundary (which the reference refuses too, src/grid_wf.c:434-437), abort with a libgrid
+    message before any GPU work."""
+    code = (
+        "import libgrid_amd as LG; lib = LG.load();"
+        f"w = lib.grid_wf_alloc(4, 4, 4, 0.1, 1.0, LG.WF_PERIODIC_BOUNDARY, {propagator}, b'w');"
+        f"{setup}"
+        "lib.grid_wf_propagate(w, None, LG.c_complex.of(0.01))"
+    )
+    p = subprocess.run(["python", "-c", code], capture_output=True, text=True, cwd=ROOT,
+                       env=dict(os.environ, HIP_VISIBLE_DEVICES="", PYTHONPATH=ROOT))
+    assert p.returncode != 0
+    assert "libgrid:" in p.stderr and message in p.stderr
