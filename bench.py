@@ -410,6 +410,8 @@ def main():
     ap.add_argument("--no-gs", action="store_true", help="skip the time-to-ground-state runs")
     ap.add_argument("--gs-max-steps", type=int, default=20000)
     ap.add_argument("--no-rows", action="store_true", help="skip the FP32 512^3, batched 2-D and propagator-variant measurements")
+    ap.add_argument("--rows", default="fp32_512,batch2d,propagators",
+                    help="comma-separated subset of the other rows to measure")
     ap.add_argument("--ws-count", type=int, default=2,
                     help="held-out 256^3 wells potentials for the warm-start time-to-ground-state comparison")
     args = ap.parse_args()
@@ -475,9 +477,13 @@ def main():
 
     rows = {}
     if not args.no_rows:
-        rows["fp32_512"] = fp32_512_bench(lib, LG)
-        rows["batch2d"] = batch2d_bench(lib, LG)
-        rows["propagators"] = propagators_bench(lib, LG)
+        which = set(args.rows.split(","))
+        if "fp32_512" in which:
+            rows["fp32_512"] = fp32_512_bench(lib, LG)
+        if "batch2d" in which:
+            rows["batch2d"] = batch2d_bench(lib, LG)
+        if "propagators" in which:
+            rows["propagators"] = propagators_bench(lib, LG)
 
     cpu = None
     if not args.no_cpu:
