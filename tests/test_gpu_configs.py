@@ -1,7 +1,8 @@
 """The workloads of the benchmark suite as GPU parity cases (full sizes).
 
   wavepacket 128^3 FP64: 100 real-time steps vs the oracle, norm every step, final FFT energy;
-  gpe barrier 256^3 FP64: the first imaginary-time GPE steps vs the oracle's composition;
+  gpe barrier 256^3 FP64: the first imaginary-time GPE steps vs the oracle's composition, and
+      its ground state by plain FP64 and by extrapolated FP32 then FP64 (same energy);
   vortex ring 512^3 FP32 (FP64 twiddles / arguments / reductions): 500 real-time GPE steps vs
       the FP64 GPU path on the same inputs (the FP64 path is pinned to the oracle above and in
       test_gpu_parity.py); reports relative L2, norm drift and energy drift.
@@ -76,6 +77,43 @@ def test_gpe_barrier_256_first_steps(lg, orc):
     assert err < 1e-12
     lg.grid_wf_free(w)
     lg.rgrid_free(vr)
+
+
+def test_gpe_barrier_256_ground_state_paths(lg):
+    """The 256^3 barrier-trap GPE relaxed to 1e-10 Eh from its seeded start: plain FP64 against
+    FP32 with vector extrapolation (to 1e-7) then plain FP64.  The two paths must reach the
+    same energy, the fast one in less time."""
+    import ctypes as C
+    import time
+
+    psi0, vext, prm = W.gpe_barrier(seed=0)
+    case(f"gpe barrier 256^3 ground state, FP64 vs extrapolated FP32 then FP64 {prm}")
+    step, g, dt, tol = prm["step"], prm["g"], prm["dt"], prm["tol"]
+    vr = lg.rgrid_alloc(256, 256, 256, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vx")
+    LG.set_rgrid(vr, vext)
+    out = {}
+    e = C.c_double()
+    for mode in ("fp64", "fp32e"):
+        w = lg.grid_wf_alloc(256, 256, 256, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"gs")
+        LG.set_cgrid(w.contents.grid, psi0)
+        lg.grid_wf_gpe_energy(w, vr, g)
+        if mode == "fp32e":
+            lg.cgrid_set_precision(w.contents.grid, b"\x01")
+        lg.grid_hip_sync()
+        t0 = time.perf_counter()
+        s32 = 0
+        if mode == "fp32e":
+            s32 = lg.grid_wf_gpe_relax_extrapolated(w, vr, g, dt, 1e-7, 200000, C.byref(e), 64, None)
+            lg.cgrid_set_precision(w.contents.grid, b"\x00")
+        s64 = lg.grid_wf_gpe_relax(w, vr, g, dt, tol, 200000, C.byref(e))
+        lg.grid_hip_sync()
+        out[mode] = (time.perf_counter() - t0, s32, s64, lg.grid_wf_gpe_energy(w, vr, g))
+        lg.grid_wf_free(w)
+    lg.rgrid_free(vr)
+    print(f"ground state: {out}")
+    assert out["fp64"][2] < 200000 and out["fp32e"][2] < 200000
+    assert abs(out["fp32e"][3] - out["fp64"][3]) < 2e-10
+    assert out["fp32e"][0] < out["fp64"][0]
 
 
 @pytest.mark.timeout(400)
