@@ -175,9 +175,19 @@ void cgrid_set_origin(cgrid *grid, REAL x0, REAL y0, REAL z0);                 /
 void cgrid_shift_origin(cgrid *grid, REAL x0, REAL y0, REAL z0);               /* :280 */
 void cgrid_set_momentum(cgrid *grid, REAL kx0, REAL ky0, REAL kz0);            /* :302 */
 void cgrid_set_rotation(cgrid *grid, REAL omega);                              /* :318 */
+/* binary grid file [INT nx, ny, nz][REAL step][nx*ny*nz complex values]; a file of another size
+ * or step is interpolated onto the grid (cgrid_extrapolate) */
 void cgrid_write(cgrid *grid, FILE *out);                                       /* :360 */
 cgrid *cgrid_read(cgrid *grid, FILE *in);                                       /* :382 */
+/* base.grd (binary, as cgrid_write) and the ASCII cuts base.x/.y/.z through the middle point */
+void cgrid_write_grid(char *base, cgrid *grid);                                 /* :432 */
+/* value(i, j, k) = func(farg, (i - nx/2) step - x0, (j - ny/2) step - y0, (k - nz/2) step - z0) */
 void cgrid_map(cgrid *grid, REAL complex (*func)(void *arg, REAL x, REAL y, REAL z), void *farg); /* :793 */
+/* the same, each value a hat-weighted average of ns^3 samples over the neighbouring cells */
+void cgrid_smooth_map(cgrid *grid, REAL complex (*func)(void *arg, REAL x, REAL y, REAL z), void *farg,
+                      INT ns);                                                  /* :832 */
+REAL complex cgrid_value(cgrid *grid, REAL x, REAL y, REAL z);                  /* :2060, trilinear */
+void cgrid_extrapolate(cgrid *dest, cgrid *src);                                /* :2126 */
 void cgrid_zero(cgrid *grid);
 void cgrid_constant(cgrid *grid, REAL complex c);
 void cgrid_copy(cgrid *dst, cgrid *src);
@@ -206,6 +216,13 @@ rgrid *rgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL (*value_outside)(rgri
                    void *outside_params_ptr, char *id);                         /* :38 */
 void rgrid_free(rgrid *grid);
 void rgrid_map(rgrid *grid, REAL (*func)(void *arg, REAL x, REAL y, REAL z), void *farg); /* :806 */
+void rgrid_smooth_map(rgrid *grid, REAL (*func)(void *arg, REAL x, REAL y, REAL z), void *farg, INT ns); /* :846 */
+/* binary grid file [INT nx, ny, nz][REAL step][nx*ny*nz2 values, the padded rows] */
+void rgrid_write(rgrid *grid, FILE *out);                                       /* :339 */
+rgrid *rgrid_read(rgrid *grid, FILE *in);                                       /* :360 */
+void rgrid_write_grid(char *base, rgrid *grid);                                 /* :505 */
+REAL rgrid_value(rgrid *grid, REAL x, REAL y, REAL z);                          /* :2208, trilinear */
+void rgrid_extrapolate(rgrid *dest, rgrid *src);                                /* :2267 */
 void rgrid_zero(rgrid *grid);
 void rgrid_constant(rgrid *grid, REAL c);
 REAL rgrid_value_at_index(rgrid *grid, INT i, INT j, INT k);
@@ -266,8 +283,9 @@ REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g);
  * estimate D2 q / (1 - q), q = D2 / D1, of the energy still to come, with two successive
  * ratios q within a factor of two; a rise smaller than tol (rounding) also ends it.  Steps run
  * in batches of 16 and only the first step of a batch, where every checkpoint falls, forms the
- * energy.  Returns the number of steps up to the deciding checkpoint (the state has run to the
- * end of its batch) and stores the energy of the last checkpoint in *energy. */
+ * energy.  Returns the number of steps applied to the state (a multiple of 16 unless max_steps
+ * ends it) and stores the energy of the returned state in *energy (may be NULL).  A non-finite
+ * energy aborts with a libgrid: message (the relaxation diverged). */
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL tol, INT max_steps, REAL *energy);
 /* grid_wf_gpe_relax with vector extrapolation (this build's addition; no reference
  * counterpart): every `stride` steps (a multiple of 16) a snapshot is taken, and when the

@@ -37,6 +37,7 @@ int hgrid_memcpy_h2d(void *dst, const void *src, size_t bytes);   /* synchronous
 int hgrid_memcpy_d2h(void *dst, const void *src, size_t bytes);   /* synchronous */
 int hgrid_memcpy_d2d(void *dst, const void *src, size_t bytes);   /* async, library stream */
 int hgrid_memset(void *ptr, int value, size_t bytes);             /* async fill kernel */
+int hgrid_copy_f64(double *dst, const double *src, long n);        /* async copy kernel (graph-safe) */
 int hgrid_sync(void);                                              /* wait for the library stream */
 /* Elapsed time of the library stream between two marks (HIP events recorded on that stream). */
 int hgrid_event_create(void **ev);
@@ -204,9 +205,10 @@ int hgrid_scalar_mul(double *s, double re, double im);
 /* s = sqrt(target / (sum[0] * dv)) (renormalisation factor from a device sum) */
 int hgrid_scalar_renorm(double *s, const double *sum, double target, double dv);
 /* FD kinetic energy integrand: out = sum Re(conj(psi) * laplace7(psi)) / h^2 (periodic,
- * Neumann or Dirichlet boundary via bc = 2/1/0). */
+ * Neumann or Dirichlet boundary via bc = 2/1/0; a Dirichlet neighbour outside the grid has the
+ * value dir_re + i dir_im, which the pending scale does not multiply). */
 int hgrid_fd_laplace_expect(double *out, const void *psi, int prec, long nx, long ny, long nz, int bc,
-                            const double *scale, double *scratch);
+                            double dir_re, double dir_im, const double *scale, double *scratch);
 
 /* HIP graphs of the library stream (no memset nodes are ever recorded: the library clears
  * device memory with kernels) */

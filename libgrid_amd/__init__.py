@@ -214,6 +214,12 @@ PROTOTYPES = {
     "cgrid_set_momentum": (V, [CG, D, D, D]),
     "cgrid_set_rotation": (V, [CG, D]),
     "cgrid_map": (V, [CG, VP, VP]),
+    "cgrid_smooth_map": (V, [CG, VP, VP, L]),
+    "cgrid_value": (CX, [CG, D, D, D]),
+    "cgrid_extrapolate": (V, [CG, CG]),
+    "cgrid_write": (V, [CG, VP]),
+    "cgrid_read": (CG, [CG, VP]),
+    "cgrid_write_grid": (V, [C.c_char_p, CG]),
     "cgrid_zero": (V, [CG]),
     "cgrid_constant": (V, [CG, CX]),
     "cgrid_copy": (V, [CG, CG]),
@@ -236,6 +242,12 @@ PROTOTYPES = {
     "rgrid_alloc": (RG, [L, L, L, D, VP, VP, C.c_char_p]),
     "rgrid_free": (V, [RG]),
     "rgrid_map": (V, [RG, VP, VP]),
+    "rgrid_smooth_map": (V, [RG, VP, VP, L]),
+    "rgrid_value": (D, [RG, D, D, D]),
+    "rgrid_extrapolate": (V, [RG, RG]),
+    "rgrid_write": (V, [RG, VP]),
+    "rgrid_read": (RG, [RG, VP]),
+    "rgrid_write_grid": (V, [C.c_char_p, RG]),
     "rgrid_zero": (V, [RG]),
     "rgrid_constant": (V, [RG, D]),
     "rgrid_value_at_index": (D, [RG, L, L, L]),

@@ -244,7 +244,12 @@ template <int N, int VAR, int NT> __device__ void solve(const hgrid_b2d_job &jb,
     // E of the normalised state that entered this step
     e = jb.tfac * t0 + s * s * jb.dA * vsum;
     if (step < jb.hist_len && threadIdx.x == 0) jb.hist[step] = e;
-    if (lg_relax_feed(&rs, step, e, jb.tol)) {
+    const int verdict = lg_relax_feed(&rs, step, e, jb.tol);
+    if (verdict < 0) {
+      n_done = -2;  // non-finite energy: the host reports the solve as diverged
+      break;
+    }
+    if (verdict > 0) {
       n_done = step;
       // finish this step so the state is whole, then stop
       double d0 = 0.0, d1 = 0.0;

@@ -191,9 +191,12 @@ __global__ void __launch_bounds__(kNT) k_sum_real(double *part, const double *a,
   block_sum_store<1>(acc, part);
 }
 
-// sum Re(conj(psi) * lap7(psi)) * (1/h^2 folded by the caller); bc 2 periodic, 1 Neumann, 0 Dirichlet
+// sum Re(conj(psi) * lap7(psi)) * (1/h^2 folded by the caller); bc 2 periodic, 1 Neumann, 0 Dirichlet.
+// The pending scale s multiplies the stored values only: the Dirichlet value stands for the
+// boundary function's return value, which the scale does not touch.
 template <typename C>
-__device__ __forceinline__ C fd_at(const C *p, long nx, long ny, long nz, long i, long j, long k, int bc, C outside) {
+__device__ __forceinline__ double2 fd_at(const C *p, long nx, long ny, long nz, long i, long j, long k, int bc,
+                                         double2 outside, double2 s) {
   if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) {
     if (bc == 2) {
       i %= nx; if (i < 0) i += nx;
@@ -211,30 +214,30 @@ __device__ __forceinline__ C fd_at(const C *p, long nx, long ny, long nz, long i
       return outside;
     }
   }
-  return p[(i * ny + j) * nz + k];
+  const C v = p[(i * ny + j) * nz + k];
+  return make_double2(s.x * (double)v.x - s.y * (double)v.y, s.x * (double)v.y + s.y * (double)v.x);
 }
 
 template <typename C>
 __global__ void __launch_bounds__(kNT) k_fd_laplace_expect(double *part, const C *p, long nx, long ny, long nz, int bc,
-                                                           const double *scale) {
+                                                           double2 outside, const double *scale) {
   double acc[1] = {0.0};
   const long n = nx * ny * nz;
-  const C zero = mk<C>(0, 0);
+  const double2 s = scale ? make_double2(scale[0], scale[1]) : make_double2(1.0, 0.0);
   LG_GRID_STRIDE(idx, n) {
     const long i = idx / (ny * nz), j = (idx / nz) % ny, k = idx % nz;
-    const C c = p[idx];
-    double lr = -6.0 * (double)c.x, li = -6.0 * (double)c.y;
-    const C nb[6] = {fd_at(p, nx, ny, nz, i, j, k + 1, bc, zero), fd_at(p, nx, ny, nz, i, j, k - 1, bc, zero),
-                     fd_at(p, nx, ny, nz, i, j + 1, k, bc, zero), fd_at(p, nx, ny, nz, i, j - 1, k, bc, zero),
-                     fd_at(p, nx, ny, nz, i + 1, j, k, bc, zero), fd_at(p, nx, ny, nz, i - 1, j, k, bc, zero)};
+    const double2 c = fd_at(p, nx, ny, nz, i, j, k, bc, outside, s);
+    const double2 nb[6] = {fd_at(p, nx, ny, nz, i, j, k + 1, bc, outside, s), fd_at(p, nx, ny, nz, i, j, k - 1, bc, outside, s),
+                           fd_at(p, nx, ny, nz, i, j + 1, k, bc, outside, s), fd_at(p, nx, ny, nz, i, j - 1, k, bc, outside, s),
+                           fd_at(p, nx, ny, nz, i + 1, j, k, bc, outside, s), fd_at(p, nx, ny, nz, i - 1, j, k, bc, outside, s)};
+    double lr = -6.0 * c.x, li = -6.0 * c.y;
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-      lr += (double)nb[m].x;
-      li += (double)nb[m].y;
+      lr += nb[m].x;
+      li += nb[m].y;
     }
-    acc[0] += (double)c.x * lr + (double)c.y * li;
+    acc[0] += c.x * lr + c.y * li;
   }
-  if (scale) acc[0] *= scale[0] * scale[0] + scale[1] * scale[1];
   block_sum_store<1>(acc, part);
 }
 
@@ -396,6 +399,13 @@ extern "C" int hgrid_stream_copy(void *d, const void *s, size_t bytes) {
   return check_launch("stream_copy");
 }
 
+__global__ void k_copy_f64(double *d, const double *s, long n) {
+  LG_GRID_STRIDE(i, n) d[i] = s[i];
+}
+
+// a kernel rather than hipMemcpyAsync, so that captured graphs hold kernel nodes only
+extern "C" int hgrid_copy_f64(double *dst, const double *src, long n) { LG_LAUNCH(k_copy_f64, n, dst, src, n); }
+
 extern "C" int hgrid_memset(void *p, int v, size_t bytes) {
   LG_LAUNCH(k_fill_bytes, (long)bytes, (unsigned char *)p, (unsigned char)v, (long)bytes);
 }
@@ -486,15 +496,16 @@ extern "C" int hgrid_sum_real(double *out, const double *a, long n, int squared,
 }
 
 extern "C" int hgrid_fd_laplace_expect(double *out, const void *psi, int prec, long nx, long ny, long nz, int bc,
-                                       const double *scale, double *scratch) {
+                                       double dir_re, double dir_im, const double *scale, double *scratch) {
   const long n = nx * ny * nz;
   const unsigned nb = grid_for(n);
+  const double2 outside = make_double2(dir_re, dir_im);
   if (prec == HGRID_C128)
     hipLaunchKernelGGL(k_fd_laplace_expect<double2>, dim3(nb), dim3(kNT), 0, stream(), scratch, (const double2 *)psi,
-                       nx, ny, nz, bc, scale);
+                       nx, ny, nz, bc, outside, scale);
   else
     hipLaunchKernelGGL(k_fd_laplace_expect<float2>, dim3(nb), dim3(kNT), 0, stream(), scratch, (const float2 *)psi,
-                       nx, ny, nz, bc, scale);
+                       nx, ny, nz, bc, outside, scale);
   if (check_launch("fd_laplace_expect")) return -1;
   return hgrid_reduce_partials(out, scratch, nb, 1);
 }

@@ -352,6 +352,8 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
     const long nn = g->ny * g->nz, o = b.off[i];
     for (long k = 0; k < nn; ++k) g->value[k] = (double)hpsi[2 * (o + k)] + I * (double)hpsi[2 * (o + k) + 1];
     cgrid_host_modified(g);
+    if (out[HGRID_B2D_NOUT * i + 2] == -2.0 || !isfinite(out[HGRID_B2D_NOUT * i]))
+      lg_fatal("batched relaxation: solve %ld (%ld x %ld) diverged: non-finite energy", (long)i, (long)g->ny, (long)g->nz);
     if (energy) energy[i] = out[HGRID_B2D_NOUT * i];
     if (steps) steps[i] = (INT)out[HGRID_B2D_NOUT * i + 2];
   }

@@ -339,49 +339,6 @@ void cgrid_value_to_index(cgrid *grid, INT i, INT j, INT k, REAL complex value) 
   grid->dev->valid = GV_HOST;
 }
 
-void cgrid_map(cgrid *grid, REAL complex (*func)(void *arg, REAL x, REAL y, REAL z), void *farg) {
-  INT nx = grid->nx, ny = grid->ny, nz = grid->nz, nxy = nx * ny;
-  REAL step = grid->step, x0 = grid->x0, y0 = grid->y0, z0 = grid->z0;
-  REAL complex *value = grid->value;
-#pragma omp parallel for schedule(static)
-  for (INT ij = 0; ij < nxy; ij++) {
-    INT ijnz = ij * nz, i = ij / ny, j = ij % ny;
-    REAL x = ((REAL)(i - nx / 2)) * step - x0;
-    REAL y = ((REAL)(j - ny / 2)) * step - y0;
-    for (INT k = 0; k < nz; k++) {
-      REAL z = ((REAL)(k - nz / 2)) * step - z0;
-      value[ijnz + k] = func(farg, x, y, z);
-    }
-  }
-  cgrid_host_modified(grid);
-}
-
-void cgrid_write(cgrid *grid, FILE *out) {
-  lg_c_host_sync(grid);
-  fwrite(&grid->nx, sizeof(INT), 1, out);
-  fwrite(&grid->ny, sizeof(INT), 1, out);
-  fwrite(&grid->nz, sizeof(INT), 1, out);
-  fwrite(&grid->step, sizeof(REAL), 1, out);
-  fwrite(grid->value, sizeof(REAL complex), (size_t)(grid->nx * grid->ny * grid->nz), out);
-}
-
-cgrid *cgrid_read(cgrid *grid, FILE *in) {
-  INT nx, ny, nz;
-  REAL step;
-  if (fread(&nx, sizeof(INT), 1, in) != 1 || fread(&ny, sizeof(INT), 1, in) != 1 || fread(&nz, sizeof(INT), 1, in) != 1 ||
-      fread(&step, sizeof(REAL), 1, in) != 1) {
-    fprintf(stderr, "libgrid: Failed to read grid header in cgrid_read().\n");
-    return NULL;
-  }
-  if (!grid) grid = cgrid_alloc(nx, ny, nz, step, CGRID_PERIODIC_BOUNDARY, NULL, "read_grid");
-  if (nx != grid->nx || ny != grid->ny || nz != grid->nz || step != grid->step)
-    lg_fatal("Grid in file has different size than grid in memory (interpolation is not supported by this build).");
-  size_t n = (size_t)(nx * ny * nz);
-  if (fread(grid->value, sizeof(REAL complex), n, in) != n) lg_fatal("Short read in cgrid_read().");
-  cgrid_host_modified(grid);
-  return grid;
-}
-
 /* ---- device operations -------------------------------------------------------------------- */
 
 void cgrid_zero(cgrid *grid) { cgrid_constant(grid, 0.0); }

@@ -657,11 +657,13 @@ REAL grid_wf_energy_fft(wf *gwf, rgrid *potential) {
   return en;
 }
 
-static int bc_code(cgrid *g) {
+/* the boundary rule of src/cgrid_bc.c:21-86 as a kernel code; *dir = the Dirichlet value */
+static int bc_code(cgrid *g, REAL complex *dir) {
+  *dir = 0.0;
   if (g->value_outside == CGRID_PERIODIC_BOUNDARY) return 2;
   if (g->value_outside == CGRID_NEUMANN_BOUNDARY) return 1;
   if (g->value_outside == CGRID_DIRICHLET_BOUNDARY) {
-    if (*((REAL complex *)g->outside_params_ptr) != 0.0) lg_fatal("non-zero Dirichlet boundary value is not supported");
+    *dir = *((REAL complex *)g->outside_params_ptr);
     return 0;
   }
   lg_fatal("user-defined boundary functions are not supported on the GPU");
@@ -673,7 +675,10 @@ REAL grid_wf_kinetic_energy_cn(wf *gwf) {
   const void *psi = lg_c_in(g, &scale);
   grid_dev *d = g->dev;
   double *scr = lg_partials(hgrid_reduce_scratch((long)d->n));
-  lg_hip(hgrid_fd_laplace_expect(d->dscal + GS_TMP, psi, d->prec, g->nx, g->ny, g->nz, bc_code(g), scale, scr),
+  REAL complex dir;
+  const int bc = bc_code(g, &dir);
+  lg_hip(hgrid_fd_laplace_expect(d->dscal + GS_TMP, psi, d->prec, g->nx, g->ny, g->nz, bc, creal(dir), cimag(dir), scale,
+                                 scr),
          "fd_laplace_expect");
   /* Re int psi^* (-hbar^2/2m) laplace(psi) (reference src/grid_wf_cn.c:37-49) */
   return -HBAR * HBAR / (2.0 * gwf->mass) * fetch(d->dscal + GS_TMP) / (g->step * g->step) *
@@ -829,6 +834,8 @@ static int extrap_snapshot(wf *gwf, extrap_state *x, double *r_out) {
   const double n11 = creal(extrap_dot(g, x->d1, x->d1));
   const double n22 = creal(extrap_dot(g, x->d2, x->d2));
   const double complex x12 = extrap_dot(g, x->d1, x->d2);
+  if (!isfinite(n11) || !isfinite(n22) || !isfinite(creal(x12)) || !isfinite(cimag(x12)))
+    lg_fatal("imaginary-time relaxation diverged: non-finite snapshot differences at an extrapolation check");
   const double r = creal(x12) / n11;
   if (n11 > 0.0 && n22 > 0.0 && cabs(x12) > LG_EXTRAP_COS * sqrt(n11 * n22) && r > 0.05 && r < 0.995) {
     *r_out = r;
@@ -844,6 +851,34 @@ static int extrap_snapshot(wf *gwf, extrap_state *x, double *r_out) {
   return 0;
 }
 
+/* The device addresses a captured relaxation batch reads or writes.  Tables and the shared
+ * reduction scratch may be reallocated by calls made between batches (an extrapolation's
+ * reductions, a precision change), so a graph is replayed only while every one of them is
+ * unchanged, and is captured again otherwise. */
+typedef struct {
+  const void *p[14];
+  unsigned long partials_gen;
+} graph_key;
+
+static void graph_key_of(wf *gwf, rgrid *vext, const double *hist, graph_key *k) {
+  grid_dev *d = gwf->grid->dev;
+  memset(k, 0, sizeof(*k));
+  int i = 0;
+  k->p[i++] = d->d;
+  k->p[i++] = d->dscal;
+  k->p[i++] = d->rho;
+  k->p[i++] = vext->dev->d;
+  k->p[i++] = hist;
+  for (int a = 0; a < 3; ++a) {
+    k->p[i++] = d->kin.t[a];
+    k->p[i++] = d->kin_energy.k2[a];
+  }
+  k->p[i++] = d->kin.kc[0];
+  k->p[i++] = d->kin.kc[1];
+  k->p[i++] = d->kin.kc[2];
+  k->partials_gen = lg_partials_generation();
+}
+
 static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy, INT stride,
                       long *n_extrap) {
   grid_dev *d = gwf->grid->dev;
@@ -857,7 +892,8 @@ static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
   lg_hip(hgrid_malloc((void **)&hist, sizeof(double) * HGRID_RED_NCHAN), "hipMalloc (energy history)");
   double host[HGRID_RED_NCHAN];
   double e = NAN;
-  INT done = -1, n = 0, n0 = 0; /* n0: step at which the stopping rule (re)started */
+  INT n = 0, n0 = 0; /* n0: step at which the stopping rule (re)started */
+  int converged = 0;
   lg_relax_state rs;
   lg_relax_init(&rs);
   extrap_state xs = {NULL, NULL, NULL, 0, 0};
@@ -867,9 +903,19 @@ static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
    * extrapolation brings the state back to real space with no pending scale, so the batch
    * after one runs eagerly and restores the captured form. */
   void *graph = NULL;
+  graph_key key, now;
   int eager = 1;
-  while (n < max_steps && done < 0) {
+  while (n < max_steps && !converged) {
     INT nb = (max_steps - n) < batch ? (max_steps - n) : batch;
+    if (graph && nb == batch && !eager) {
+      graph_key_of(gwf, vext, hist, &now);
+      if (memcmp(&now, &key, sizeof(key))) {
+        /* a buffer the graph holds has moved: drop it and run this batch eagerly */
+        lg_hip(hgrid_graph_free(graph), "hipGraphExecDestroy");
+        graph = NULL;
+        eager = 1;
+      }
+    }
     if (graph && nb == batch && !eager) {
       lg_hip(hgrid_graph_launch(graph), "hipGraphLaunch (relaxation)");
     } else {
@@ -879,21 +925,23 @@ static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
         /* the rule's checkpoints fall on multiples of its strides, all multiples of 16 from
          * its start, so only the first step of a batch computes its input state's energy */
         gpe_step(gwf, vext, g_nl, -I * dt, 1, b == 0);
-        if (b == 0)
-          lg_hip(hgrid_memcpy_d2d(hist, d->dscal + GS_RED, sizeof(double) * HGRID_RED_NCHAN),
-                 "hipMemcpy D2D (energy history)");
+        if (b == 0) lg_hip(hgrid_copy_f64(hist, d->dscal + GS_RED, HGRID_RED_NCHAN), "copy (energy history)");
       }
       if (capture) {
         lg_hip(hgrid_graph_end(&graph), "hipStreamEndCapture (relaxation)");
+        graph_key_of(gwf, vext, hist, &key);
         lg_hip(hgrid_graph_launch(graph), "hipGraphLaunch (relaxation)"); /* capture does not execute */
       }
       eager = 0;
     }
     lg_hip(hgrid_memcpy_d2h(host, hist, sizeof(double) * HGRID_RED_NCHAN), "hipMemcpy D2H (energy history)");
     e = gpe_energy_from_parts(gwf, host, g_nl); /* the energy of the state entering step n */
-    if (lg_relax_feed(&rs, (long)(n - n0), e, tol)) done = n + 1;
+    const int verdict = lg_relax_feed(&rs, (long)(n - n0), e, tol);
+    if (verdict < 0)
+      lg_fatal("imaginary-time relaxation diverged: the energy of the state entering step %ld is %g", (long)n, e);
+    converged = verdict > 0;
     n += nb;
-    if (done < 0 && stride && nb == batch && n % stride == 0) {
+    if (!converged && stride && nb == batch && n % stride == 0) {
       double r = 0.0;
       if (extrap_snapshot(gwf, &xs, &r)) {
         lg_relax_init(&rs);
@@ -914,8 +962,13 @@ static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
     lg_hip(hgrid_free(xs.d2), "hipFree");
   }
   if (n_extrap) *n_extrap = xs.count;
-  if (energy) *energy = e;
-  return done < 0 ? n : done;
+  /* the energy of the state returned (the stopping rule decided on a checkpoint up to 15
+   * steps earlier) */
+  if (energy) {
+    *energy = grid_wf_gpe_energy(gwf, vext, g_nl);
+    if (!isfinite(*energy)) lg_fatal("imaginary-time relaxation diverged: final energy %g", *energy);
+  }
+  return n;
 }
 
 INT grid_wf_gpe_relax(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT max_steps, REAL *energy) {

@@ -55,6 +55,7 @@ void lg_fatal(const char *fmt, ...) __attribute__((noreturn, format(printf, 1, 2
 void lg_ensure_device(void);
 void lg_hip(int rc, const char *what);          /* abort with the HIP error text if rc != 0 */
 double *lg_partials(long ndoubles);             /* library reduction scratch (device) */
+unsigned long lg_partials_generation(void);     /* changes whenever lg_partials reallocates */
 const void *lg_twiddles(long n, int prec);      /* device table exp(-2 pi i m / n), m < n */
 size_t lg_prec_size(int prec);
 

@@ -65,8 +65,10 @@ LG_RELAX_FN void lg_relax_init(lg_relax_state *s) {
 }
 
 /* Feed E_n, the energy of the state entering step n (n = 0, 1, 2, ...); returns 1 once the
- * energy is converged to tol. */
+ * energy is converged to tol, and -1 at once if E_n is not finite (the relaxation diverged: the
+ * counterpart of the reference's floating-point exception trap, src/threads.c:122-126). */
 LG_RELAX_FN int lg_relax_feed(lg_relax_state *s, long n, double e, double tol) {
+  if (!isfinite(e)) return -1;
   if (n != s->next) return 0;
   s->next = n + s->m;
   if (s->nchk < 2) {

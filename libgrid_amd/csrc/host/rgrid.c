@@ -161,23 +161,6 @@ void rgrid_value_to_index(rgrid *grid, INT i, INT j, INT k, REAL value) {
   grid->dev->valid = GV_HOST;
 }
 
-void rgrid_map(rgrid *grid, REAL (*func)(void *arg, REAL x, REAL y, REAL z), void *farg) {
-  INT nx = grid->nx, ny = grid->ny, nxy = nx * ny, nz = grid->nz, nzz = grid->nz2;
-  REAL step = grid->step, x0 = grid->x0, y0 = grid->y0, z0 = grid->z0;
-  REAL *value = grid->value;
-#pragma omp parallel for schedule(static)
-  for (INT ij = 0; ij < nxy; ij++) {
-    INT ijnz = ij * nzz, i = ij / ny, j = ij % ny;
-    REAL x = ((REAL)(i - nx / 2)) * step - x0;
-    REAL y = ((REAL)(j - ny / 2)) * step - y0;
-    for (INT k = 0; k < nz; k++) {
-      REAL z = ((REAL)(k - nz / 2)) * step - z0;
-      value[ijnz + k] = func(farg, x, y, z);
-    }
-  }
-  grid->dev->valid = GV_HOST;
-}
-
 void rgrid_zero(rgrid *grid) { rgrid_constant(grid, 0.0); }
 
 void rgrid_constant(rgrid *grid, REAL c) {

@@ -64,6 +64,7 @@ size_t lg_prec_size(int prec) { return prec == HGRID_C128 ? 16 : 8; }
 
 static double *g_partials = NULL;
 static long g_partials_n = 0;
+static unsigned long g_partials_gen = 0; /* counts reallocations: captured graphs hold the address */
 
 double *lg_partials(long ndoubles) {
   lg_ensure_device();
@@ -72,9 +73,12 @@ double *lg_partials(long ndoubles) {
     long n = ndoubles < 65536 ? 65536 : ndoubles;
     lg_hip(hgrid_malloc((void **)&g_partials, (size_t)n * sizeof(double)), "hipMalloc (reduction scratch)");
     g_partials_n = n;
+    ++g_partials_gen;
   }
   return g_partials;
 }
+
+unsigned long lg_partials_generation(void) { return g_partials_gen; }
 
 /* ---- twiddle tables ----------------------------------------------------------------------- */
 

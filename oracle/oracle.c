@@ -451,13 +451,40 @@ double or_energy_fft(const cplx *a, const double *v, long nx, long ny, long nz, 
   return e;
 }
 
-/* finite-difference kinetic energy, grid_wf_kinetic_energy_cn (src/grid_wf_cn.c:37-49) with
- * cgrid_fd_laplace (src/cgrid_diff.c:303-330) and the periodic rule of src/cgrid_bc.c:69-86 */
-double or_kinetic_energy_fd_periodic(const cplx *a, long nx, long ny, long nz, double step, double mass) {
+/* The value at (i, j, k) under a grid's boundary function (src/cgrid.c:1960-1965 with
+ * src/cgrid_bc.c:21-86): bc 2 periodic (wrap at 0 and n), 1 Neumann (mirror about 0 and n - 1;
+ * a mirrored index still outside, possible only for n < 3, falls back to the Dirichlet value),
+ * 0 Dirichlet (the constant dir). */
+static cplx fd_value(const cplx *a, long nx, long ny, long nz, long i, long j, long k, int bc, cplx dir) {
+  if (i >= 0 && j >= 0 && k >= 0 && i < nx && j < ny && k < nz) return a[(i * ny + j) * nz + k];
+  if (bc == 2) {
+    i %= nx; if (i < 0) i += nx;
+    j %= ny; if (j < 0) j += ny;
+    k %= nz; if (k < 0) k += nz;
+    return a[(i * ny + j) * nz + k];
+  }
+  if (bc == 1) {
+    if (i < 0) i = -i;
+    if (i > nx - 1) i = 2 * nx - i - 2;
+    if (j < 0) j = -j;
+    if (j > ny - 1) j = 2 * ny - j - 2;
+    if (k < 0) k = -k;
+    if (k > nz - 1) k = 2 * nz - k - 2;
+    if (i >= 0 && j >= 0 && k >= 0 && i < nx && j < ny && k < nz) return a[(i * ny + j) * nz + k];
+  }
+  return dir;
+}
+
+/* finite-difference kinetic energy, grid_wf_kinetic_energy_cn (src/grid_wf_cn.c:37-49):
+ * Re int conj(psi) (-hbar^2/2m) lap7(psi), with cgrid_fd_laplace (src/cgrid_diff.c:303-330) and
+ * the boundary rule bc of fd_value; dir_re + i dir_im is the Dirichlet value. */
+double or_kinetic_energy_fd(const cplx *a, long nx, long ny, long nz, double step, double mass, int bc, double dir_re,
+                            double dir_im) {
   long nxy = nx * ny;
   double inv_delta2 = 1.0 / (step * step);
+  const cplx dir = dir_re + I * dir_im;
   double *rows = (double *)malloc(sizeof(double) * (size_t)nxy);
-#define AT(i, j, k) a[((((i) % nx + nx) % nx) * ny + (((j) % ny + ny) % ny)) * nz + (((k) % nz + nz) % nz)]
+#define AT(i, j, k) fd_value(a, nx, ny, nz, (i), (j), (k), bc, dir)
 #pragma omp parallel for schedule(static)
   for (long ij = 0; ij < nxy; ++ij) {
     long i = ij / ny, j = ij % ny;
@@ -475,6 +502,10 @@ double or_kinetic_energy_fd_periodic(const cplx *a, long nx, long ny, long nz, d
   for (long ij = 0; ij < nxy; ++ij) kadd(&k, rows[ij]);
   free(rows);
   return (k.s + k.c) * step_factor(nx, ny, nz, step);
+}
+
+double or_kinetic_energy_fd_periodic(const cplx *a, long nx, long ny, long nz, double step, double mass) {
+  return or_kinetic_energy_fd(a, nx, ny, nz, step, mass, 2, 0.0, 0.0);
 }
 
 /* ---- Gross-Pitaevskii composition of reference calls ---------------------------------------

@@ -45,6 +45,7 @@ _PROTOS = {
     "or_kinetic_energy_fft": (D, [P, L, L, L, D, D]),
     "or_energy_fft": (D, [P, P, L, L, L, D, D]),
     "or_kinetic_energy_fd_periodic": (D, [P, L, L, L, D, D]),
+    "or_kinetic_energy_fd": (D, [P, L, L, L, D, D, I, D, D]),
     "or_gpe_step": (None, [P, P, D, L, L, L, D, D, D, D, D, D, D, I, D, D]),
     "or_gpe_energy": (D, [P, P, D, L, L, L, D, D]),
     "or_num_threads": (I, []),
@@ -217,6 +218,108 @@ def energy_fft(a, v, step, mass) -> float:
 def kinetic_energy_fd_periodic(a, step, mass) -> float:
     a = _c(a)
     return lib().or_kinetic_energy_fd_periodic(a.ctypes.data, *a.shape, step, mass)
+
+
+# boundary rules of the finite-difference energy (src/cgrid_bc.c:21-86)
+BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = 0, 1, 2
+
+
+def kinetic_energy_fd(a, step, mass, bc, dirichlet=0j) -> float:
+    """grid_wf_kinetic_energy_cn with the given boundary rule and Dirichlet value."""
+    a = _c(a)
+    d = complex(dirichlet)
+    return lib().or_kinetic_energy_fd(a.ctypes.data, *a.shape, step, mass, bc, d.real, d.imag)
+
+
+# ---- host-side grid mapping and interpolation (numpy restatements) ---------------------------
+
+def axis_coords(n, step, origin=0.0):
+    """Coordinates of the n points of an axis: (i - n/2) step - origin (src/cgrid.c:793-815)."""
+    return (np.arange(n, dtype=np.int64) - n // 2).astype(np.float64) * step - origin
+
+
+def cell_rule(step, ns):
+    """Offsets and hat weights of the ns samples per axis of cgrid_smooth_map
+    (linearly_weighted_integralc, src/cprivate.h:35-68)."""
+    sub = 2.0 * step / float(ns)
+    off = np.array([-step + (float(m) + 0.5) * sub for m in range(ns)])
+    return off, 1.0 - np.abs(off / step)
+
+
+def smooth_map(func, shape, step, origin=(0.0, 0.0, 0.0), ns=1):
+    """cgrid_smooth_map / rgrid_smooth_map (src/cgrid.c:832-860, src/rgrid.c:846-870): func is
+    vectorised over coordinate arrays; each value is sum_m w_m f(r + s_m) / sum_m w_m over the
+    ns^3 samples, accumulated in the reference's loop order (x samples outermost)."""
+    x = axis_coords(shape[0], step, origin[0])[:, None, None]
+    y = axis_coords(shape[1], step, origin[1])[None, :, None]
+    z = axis_coords(shape[2], step, origin[2])[None, None, :]
+    if ns == 1:
+        return np.broadcast_to(func(x, y, z), shape).copy()
+    off, w = cell_rule(step, ns)
+    acc = 0.0
+    wacc = 0.0
+    for a in range(ns):
+        for b in range(ns):
+            for c in range(ns):
+                wt = w[a] * w[b] * w[c]
+                acc = acc + wt * func(x + off[a], y + off[b], z + off[c])
+                wacc += wt
+    return np.broadcast_to(acc / wacc, shape).copy()
+
+
+def _axis_cell(u, origin, step, n):
+    """Lower cell index (shifted by n/2) and fraction of coordinate u (src/cgrid.c:2070-2090)."""
+    t = (u + origin) / step
+    c = np.trunc(t).astype(np.int64)
+    c = np.where(t < 0, c - 1, c)
+    return c + n // 2, t - c
+
+
+def value_at(a, i, j, k, bc, dirichlet=0.0):
+    """a[i, j, k] with the boundary function's value outside the grid (src/cgrid_bc.c:21-86);
+    a Neumann mirror that stays outside (n < 3) takes the Dirichlet value."""
+    nx, ny, nz = a.shape
+    i, j, k = np.broadcast_arrays(np.asarray(i), np.asarray(j), np.asarray(k))
+    if bc == BC_PERIODIC:
+        return a[i % nx, j % ny, k % nz]
+    if bc == BC_NEUMANN:
+        def mirror(m, n):
+            m = np.where(m < 0, -m, m)
+            return np.where(m > n - 1, 2 * n - m - 2, m)
+        i, j, k = mirror(i, nx), mirror(j, ny), mirror(k, nz)
+    inside = (i >= 0) & (j >= 0) & (k >= 0) & (i < nx) & (j < ny) & (k < nz)
+    out = np.full(i.shape, dirichlet, dtype=a.dtype)
+    out[inside] = a[i[inside], j[inside], k[inside]]
+    return out
+
+
+def interpolate(a, step, origin, x, y, z, bc, dirichlet=0.0):
+    """cgrid_value (complex a, nested weights, src/cgrid.c:2060-2104) or rgrid_value (real a,
+    expanded weights, src/rgrid.c:2208-2256) at coordinate arrays x, y, z."""
+    nx, ny, nz = a.shape
+    i, fx = _axis_cell(np.asarray(x, dtype=np.float64), origin[0], step, nx)
+    j, fy = _axis_cell(np.asarray(y, dtype=np.float64), origin[1], step, ny)
+    k, fz = _axis_cell(np.asarray(z, dtype=np.float64), origin[2], step, nz)
+    gx, gy, gz = 1.0 - fx, 1.0 - fy, 1.0 - fz
+
+    def f(di, dj, dk):
+        return value_at(a, i + di, j + dj, k + dk, bc, dirichlet)
+
+    if np.iscomplexobj(a):
+        return gx * (gy * (gz * f(0, 0, 0) + fz * f(0, 0, 1)) + fy * (gz * f(0, 1, 0) + fz * f(0, 1, 1))) + \
+            fx * (gy * (gz * f(1, 0, 0) + fz * f(1, 0, 1)) + fy * (gz * f(1, 1, 0) + fz * f(1, 1, 1)))
+    return gx * gy * gz * f(0, 0, 0) + fx * gy * gz * f(1, 0, 0) + gx * fy * gz * f(0, 1, 0) + gx * gy * fz * f(0, 0, 1) + \
+        fx * fy * gz * f(1, 1, 0) + fx * gy * fz * f(1, 0, 1) + gx * fy * fz * f(0, 1, 1) + fx * fy * fz * f(1, 1, 1)
+
+
+def extrapolate(src, src_step, dst_shape, dst_step, dst_origin=(0.0, 0.0, 0.0), src_origin=(0.0, 0.0, 0.0), bc=BC_PERIODIC):
+    """cgrid_extrapolate / rgrid_extrapolate (src/cgrid.c:2126-2150, src/rgrid.c:2267-2288): the
+    source interpolated at every destination coordinate."""
+    x = axis_coords(dst_shape[0], dst_step, dst_origin[0])[:, None, None]
+    y = axis_coords(dst_shape[1], dst_step, dst_origin[1])[None, :, None]
+    z = axis_coords(dst_shape[2], dst_step, dst_origin[2])[None, None, :]
+    x, y, z = np.broadcast_arrays(x, y, z)
+    return interpolate(src, src_step, src_origin, x, y, z, bc)
 
 
 def gpe_step_inplace(b, vext, g, step, mass, k0, time, normalize=True, target=1.0, kmax=None) -> None:

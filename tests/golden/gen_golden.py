@@ -9,7 +9,9 @@ import os
 
 import numpy as np
 
-SHAPES = [(16, 16, 16), (8, 4, 2), (6, 10, 3), (1, 12, 9), (1, 96, 64), (1, 1, 64), (4, 32, 8), (2, 2, 256)]
+# appended shapes keep the seeds (1000 + position) of the earlier ones
+SHAPES = [(16, 16, 16), (8, 4, 2), (6, 10, 3), (1, 12, 9), (1, 96, 64), (1, 1, 64), (4, 32, 8), (2, 2, 256),
+          (12, 48, 6), (1, 192, 24), (3, 1, 384)]
 
 
 def main():

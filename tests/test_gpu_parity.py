@@ -53,8 +53,15 @@ def rand_c(shape, seed):
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex128)
 
 
+# powers of two up to 1024 and 3 * 2^k up to 384 on every axis, in 1-, 2- and 3-D shapes
 FFT_SHAPES = [(16, 16, 16), (8, 4, 2), (4, 32, 8), (2, 2, 256), (1, 1, 64), (64, 64, 64), (32, 64, 128), (512, 4, 2),
-              (2, 1024, 2), (2, 16, 1), (1, 1, 1), (128, 1, 1), (1, 256, 256)]
+              (2, 1024, 2), (2, 16, 1), (1, 1, 1), (128, 1, 1), (1, 256, 256),
+              (48, 96, 192), (1, 192, 192), (2, 384, 2), (3, 6, 12), (24, 1, 96), (384, 2, 3), (96, 64, 48)]
+
+
+def gpu_length(n):
+    """Axis lengths the pencil transforms take: powers of two up to 1024, 3 * 2^k up to 384."""
+    return n == 1 or (n <= 1024 and (n & (n - 1)) == 0) or (n % 3 == 0 and n <= 384 and ((n // 3) & (n // 3 - 1)) == 0)
 
 
 @pytest.mark.parametrize("shape", FFT_SHAPES)
@@ -77,8 +84,8 @@ def test_fft_golden_vectors(lg):
     gold = np.load(GOLDEN)
     for tag in sorted(k[3:] for k in gold.files if k.startswith("in_")):
         shape = tuple(int(s) for s in tag.split("x"))
-        if not all(n == 1 or (n & (n - 1)) == 0 for n in shape):
-            continue  # radix-3/5 lengths are not in the GPU transform set yet
+        if not all(gpu_length(n) for n in shape):
+            continue  # lengths with a factor 5 or 9 (6x10x3, 1x12x9) are outside the GPU transform set
         case(f"golden {tag}")
         g = new_cgrid(lg, gold["in_" + tag])
         lg.cgrid_fft(g)
@@ -87,6 +94,66 @@ def test_fft_golden_vectors(lg):
         lg.cgrid_inverse_fft(g)
         assert rel_l2(LG.get_cgrid(g), gold["bwd_" + tag]) < TOL, tag
         lg.cgrid_free(g)
+
+
+def test_fft_length_set_matches_library(lg):
+    """The lengths the tests treat as supported are exactly those the library accepts."""
+    lg.hgrid_fft_supported.restype = C.c_int
+    lg.hgrid_fft_supported.argtypes = [C.c_long]
+    for n in range(2, 1100):
+        assert bool(lg.hgrid_fft_supported(n)) == gpu_length(n), n
+
+
+@pytest.mark.parametrize("shape", [(8, 16, 12), (1, 48, 64), (1, 1, 96)])
+def test_inverse_fft_norm2(lg, orc, shape):
+    """cgrid_inverse_fft_norm2 (src/cgrid.c:1887-1890): the inverse transform scaled by
+    fft_norm2 = step^d / N, d = number of axes longer than 1."""
+    case(f"inverse_fft_norm2 {shape}")
+    step = 0.3
+    a = rand_c(shape, 17)
+    g = new_cgrid(lg, a, step)
+    d = sum(1 for n in shape if n > 1)
+    assert abs(g.contents.fft_norm2 - step**d / a.size) < 1e-18
+    lg.cgrid_inverse_fft_norm2(g)
+    assert rel_l2(LG.get_cgrid(g), orc.fft3d(a, +1) * (step**d / a.size)) < TOL
+    # after a forward transform it is the spatially normalised round trip
+    LG.set_cgrid(g, a)
+    lg.cgrid_fft(g)
+    lg.cgrid_inverse_fft_norm2(g)
+    assert rel_l2(LG.get_cgrid(g), a * step**d) < TOL
+    lg.cgrid_free(g)
+
+
+@pytest.mark.parametrize("boundary,dirichlet", [("neumann", 0j), ("dirichlet", 0j), ("dirichlet", 0.3 - 0.2j),
+                                                ("periodic", 0j)])
+def test_fd_energy_boundary_rules(lg, orc, boundary, dirichlet):
+    """grid_wf_energy's finite-difference branch (src/grid_wf_cn.c:19-49 with the 7-point
+    Laplacian of src/cgrid_diff.c:303-330) under each boundary function of src/cgrid_bc.c:21-86,
+    with a pending scale on the grid, against the oracle."""
+    case(f"fd energy {boundary} dirichlet={dirichlet}")
+    shape, step, mass = (12, 10, 14), 0.3, 1.3
+    a = rand_c(shape, 27) + 0.5
+    v = np.random.default_rng(28).uniform(0, 1, shape)
+    bcs = {"neumann": ("cgrid_value_outside_neumann", 1), "dirichlet": ("cgrid_value_outside_dirichlet", 0),
+           "periodic": ("cgrid_value_outside_periodic", 2)}
+    fname, bc = bcs[boundary]
+    w = lg.grid_wf_alloc(*shape, step, mass, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"fd")
+    dval = (C.c_double * 2)(dirichlet.real, dirichlet.imag)
+    w.contents.grid.contents.value_outside = LG.boundary(fname)
+    if dirichlet != 0:
+        w.contents.grid.contents.outside_params_ptr = C.cast(dval, C.c_void_p)
+    LG.set_cgrid(w.contents.grid, a)
+    s = 0.8 + 0.6j
+    lg.cgrid_multiply(w.contents.grid, LG.c_complex.of(s))  # pending scale: multiplies the values, not the boundary value
+    vr = new_rgrid(lg, v, step)
+    lg.grid_wf_analyze_method(b"\x00")
+    e = lg.grid_wf_energy(w, vr)
+    ref = orc.kinetic_energy_fd(a * s, step, mass, bc, dirichlet) + orc.grid_expectation(a * s, v, step)
+    assert abs(e - ref) < 1e-12 * abs(ref)
+    t = lg.grid_wf_kinetic_energy_cn(w)
+    assert abs(t - orc.kinetic_energy_fd(a * s, step, mass, bc, dirichlet)) < 1e-12 * abs(ref)
+    lg.grid_wf_free(w)
+    lg.rgrid_free(vr)
 
 
 def test_scaled_inverse_and_lazy_multiply(lg, orc):

@@ -426,3 +426,147 @@ def test_gpe_g0_equals_plain_step(orc):
     orc.gpe_step_inplace(a, v, 0.0, prm["step"], 1.0, (0, 0, 0), -0.05j, normalize=True)
     b, _ = orc.wf_normalize(orc.wf_propagate(psi, v.astype(complex), prm["step"], 1.0, (0, 0, 0), -0.05j), prm["step"])
     assert rel_l2(a, b) < 1e-15
+
+
+def _gauss_overlap_1d(xa, wa, pa, xb, wb, pb):
+    """int conj(exp(-(x-xa)^2/wa^2 + i pa (x-xa))) exp(-(x-xb)^2/wb^2 + i pb (x-xb)) dx in closed
+    form: exp(-al x^2 + be x + ga) integrates to sqrt(pi/al) exp(be^2/(4 al) + ga)."""
+    al = 1 / wa**2 + 1 / wb**2
+    be = 2 * xa / wa**2 + 2 * xb / wb**2 + 1j * (pb - pa)
+    ga = -xa**2 / wa**2 - xb**2 / wb**2 + 1j * (pa * xa - pb * xb)
+    return np.sqrt(np.pi / al) * np.exp(be * be / (4 * al) + ga)
+
+
+def test_conjugate_product_closed_form_and_vdot(orc):
+    """or_integral_of_conjugate_product (src/cgrid.c:1640-1658) is step^d sum conj(a) b: against
+    np.vdot on random 3-D, 2-D and 1-D grids, and against the closed-form overlap of two Gaussian
+    packets with different centres, widths and momenta (its imaginary part changes sign if the
+    conjugate is taken of the second factor, and the step^d factor shows in the modulus)."""
+    rng = np.random.default_rng(41)
+    for shape, step in (((6, 10, 12), 0.3), ((1, 16, 9), 0.25), ((1, 1, 33), 0.5)):
+        a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        b = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d = sum(1 for n in shape if n > 1)
+        ref = np.vdot(a, b) * step**d
+        got = orc.integral_of_conjugate_product(a, b, step)
+        assert abs(got - ref) < 1e-13 * abs(ref), shape
+        assert abs(orc.integral_of_conjugate_product(b, a, step) - np.conj(ref)) < 1e-13 * abs(ref)
+    n, step = 96, 0.2
+    ra, wa, pa = np.array([0.5, -0.8, 0.3]), 1.3, np.array([0.7, -0.2, 0.4])
+    rb, wb, pb = np.array([-0.4, 0.2, 0.9]), 1.6, np.array([-0.3, 0.5, 0.1])
+    psi_a = W.gaussian_packet((n, n, n), step, wa, ra, pa)
+    psi_b = W.gaussian_packet((n, n, n), step, wb, rb, pb)
+    exact = np.prod([_gauss_overlap_1d(ra[m], wa, pa[m], rb[m], wb, pb[m]) for m in range(3)])
+    got = orc.integral_of_conjugate_product(psi_a, psi_b, step)
+    assert abs(exact.imag) > 0.1 * abs(exact)  # the sign of the imaginary part is tested
+    assert abs(got - exact) < 1e-12 * abs(exact)
+
+
+def _fd_eigen_state(shape, kind, modes, phase=0.3):
+    """Exact eigenvectors of the 3-point second difference under each boundary rule, and their
+    eigenvalue factor sum_axes (2 cos theta - 2): periodic exp(2 pi i a m / n); Neumann (mirror
+    about 0 and n-1) cos(pi a m / (n-1)); zero Dirichlet sin(pi a (m+1) / (n+1))."""
+    f = np.ones(shape, dtype=np.complex128) * np.exp(1j * phase)
+    lam = 0.0
+    for axis, (n, a) in enumerate(zip(shape, modes)):
+        m = np.arange(n)
+        if kind == "periodic":
+            th = 2 * np.pi * a / n
+            v = np.exp(1j * th * m)
+        elif kind == "neumann":
+            th = np.pi * a / (n - 1)
+            v = np.cos(th * m)
+        else:
+            th = np.pi * a / (n + 1)
+            v = np.sin(th * (m + 1))
+        lam += 2 * np.cos(th) - 2
+        sh = [1, 1, 1]
+        sh[axis] = n
+        f = f * v.reshape(sh)
+    return f, lam
+
+
+@pytest.mark.parametrize("kind,bc", [("periodic", 2), ("neumann", 1), ("dirichlet", 0)])
+def test_fd_kinetic_energy_boundary_eigenstates(orc, kind, bc):
+    """The FD energy under each boundary rule of src/cgrid_bc.c:21-86 on an exact eigenvector of
+    the 7-point Laplacian with that rule: T = -(hbar^2 / 2m) (lambda / h^2) int |psi|^2.  A mirror
+    about -1/2 instead of 0 (Neumann) or a wrap for Dirichlet would break the eigenvector."""
+    shape, step, mass = (9, 12, 7), 0.35, 1.4
+    f, lam = _fd_eigen_state(shape, kind, (2, 3, 1))
+    exact = -1.0 / (2 * mass) * lam / step**2 * np.sum(np.abs(f) ** 2) * step**3
+    got = orc.kinetic_energy_fd(f, step, mass, bc)
+    assert abs(got - exact) < 1e-12 * abs(exact), (kind, got, exact)
+    # the periodic wrapper is the periodic rule
+    if bc == 2:
+        assert orc.kinetic_energy_fd_periodic(f, step, mass) == got
+
+
+def test_fd_kinetic_energy_dirichlet_value(orc):
+    """A non-zero Dirichlet value d (src/cgrid_bc.c:21-24): d + s with s a zero-Dirichlet
+    eigenvector has Laplacian lambda s, so T = -(hbar^2/2m)(lambda/h^2)(Re(conj(d) sum s) +
+    sum |s|^2) h^3; a constant d has T = 0."""
+    shape, step, mass, d = (8, 6, 10), 0.3, 1.0, 0.4 - 0.7j
+    s, lam = _fd_eigen_state(shape, "dirichlet", (1, 2, 3))
+    f = s + d
+    exact = -1.0 / (2 * mass) * lam / step**2 * (np.real(np.conj(d) * s.sum()) + np.sum(np.abs(s) ** 2)) * step**3
+    assert abs(orc.kinetic_energy_fd(f, step, mass, 0, d) - exact) < 1e-12 * abs(exact)
+    const = np.full(shape, d)
+    assert abs(orc.kinetic_energy_fd(const, step, mass, 0, d)) < 1e-13
+    # with the default value 0 the constant grid has a boundary jump: not zero
+    assert abs(orc.kinetic_energy_fd(const, step, mass, 0, 0.0)) > 1.0
+
+
+def test_interpolation_oracle_pins(orc):
+    """The trilinear interpolation of cgrid_value / rgrid_value (src/cgrid.c:2060-2104,
+    src/rgrid.c:2208-2256) against scipy's RegularGridInterpolator inside the grid, exact
+    reproduction of a linear function, the grid values at grid points (including negative integer
+    coordinates, where the reference's floor takes the lower cell with fraction 1) and the
+    periodic wrap between the last and the first point."""
+    from scipy.interpolate import RegularGridInterpolator
+
+    rng = np.random.default_rng(5)
+    shape, step, origin = (7, 8, 6), 0.5, (0.2, -0.1, 0.3)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    r = rng.standard_normal(shape)
+    axes = [orc.axis_coords(n, step, o) for n, o in zip(shape, origin)]
+    pts = np.stack([rng.uniform(ax[0] + 1e-9, ax[-1] - 1e-9, 200) for ax in axes], axis=1)
+    for vals in (a, r):
+        got = orc.interpolate(vals, step, origin, pts[:, 0], pts[:, 1], pts[:, 2], orc.BC_PERIODIC)
+        if np.iscomplexobj(vals):
+            want = RegularGridInterpolator(axes, vals.real)(pts) + 1j * RegularGridInterpolator(axes, vals.imag)(pts)
+        else:
+            want = RegularGridInterpolator(axes, vals)(pts)
+        assert np.max(np.abs(got - want)) < 1e-13
+    x, y, z = np.meshgrid(*axes, indexing="ij")
+    lin = 0.3 + 0.7 * x - 1.1 * y + 0.4 * z
+    got = orc.interpolate(lin, step, origin, pts[:, 0], pts[:, 1], pts[:, 2], orc.BC_PERIODIC)
+    assert np.max(np.abs(got - (0.3 + 0.7 * pts[:, 0] - 1.1 * pts[:, 1] + 0.4 * pts[:, 2]))) < 1e-13
+    at_nodes = orc.interpolate(a, step, origin, x, y, z, orc.BC_PERIODIC)
+    assert np.max(np.abs(at_nodes - a)) < 1e-14
+    # periodic wrap: halfway past the last x point lies between x = nx-1 and x = 0
+    xw = axes[0][-1] + 0.5 * step
+    got = orc.interpolate(r, step, origin, np.array([xw]), np.array([axes[1][2]]), np.array([axes[2][3]]), orc.BC_PERIODIC)
+    assert abs(got[0] - 0.5 * (r[-1, 2, 3] + r[0, 2, 3])) < 1e-14
+    # Neumann: halfway before the first x point lies between x = 1 (mirror of -1) and x = 0
+    xn = axes[0][0] - 0.5 * step
+    got = orc.interpolate(r, step, origin, np.array([xn]), np.array([axes[1][2]]), np.array([axes[2][3]]), orc.BC_NEUMANN)
+    assert abs(got[0] - 0.5 * (r[1, 2, 3] + r[0, 2, 3])) < 1e-14
+
+
+@pytest.mark.parametrize("ns,shift", [(1, 0.0), (2, 0.75), (3, 8.0 / 15.0)])
+def test_smooth_map_oracle_closed_form(orc, ns, shift):
+    """cgrid_smooth_map's cell average (src/cprivate.h:35-68): ns = 1 is the point value, a linear
+    function averages to its centre value (the hat weights are symmetric), and r^2 averages to
+    r^2 + 3 <s^2>, where <s^2> = h^2/4 for ns = 2 and 8 h^2/45 for ns = 3."""
+    shape, step, origin = (5, 6, 4), 0.4, (0.1, 0.0, -0.2)
+    x, y, z = np.meshgrid(*[orc.axis_coords(n, step, o) for n, o in zip(shape, origin)], indexing="ij")
+
+    def quad(x, y, z):
+        return x * x + y * y + z * z + 0j
+
+    def lin(x, y, z):
+        return 0.5 + 0.2 * x - 0.3 * y + 0.9 * z + 1j * (0.1 * x + 0.4 * z)
+
+    assert np.max(np.abs(orc.smooth_map(lin, shape, step, origin, ns) - lin(x, y, z))) < 1e-14
+    got = orc.smooth_map(quad, shape, step, origin, ns)
+    assert np.max(np.abs(got - (x * x + y * y + z * z + shift * step * step))) < 1e-14

@@ -14,22 +14,27 @@ HDR = os.path.join(HERE, "..", "libgrid_amd", "csrc", "host")
 
 SRC = r"""
 #include "relax_rule.h"
+/* the step at which the rule stops, -1 if it never does, -2 - k if it reports divergence at k */
+static long feed_all(lg_relax_state *s, const double *e, long n, double tol) {
+  for (long k = 0; k < n; ++k) {
+    const int v = lg_relax_feed(s, k, e[k], tol);
+    if (v < 0) return -2 - k;
+    if (v) return k;
+  }
+  return -1;
+}
 long run_floor(const double *e, long n, double tol, int rise_exit, double lnq_floor) {
   lg_relax_state s;
   lg_relax_init(&s);
   s.rise_exit = rise_exit;
   s.lnq_floor = lnq_floor;
-  for (long k = 0; k < n; ++k)
-    if (lg_relax_feed(&s, k, e[k], tol)) return k;
-  return -1;
+  return feed_all(&s, e, n, tol);
 }
 long run_ex(const double *e, long n, double tol, int rise_exit) {
   lg_relax_state s;
   lg_relax_init(&s);
   s.rise_exit = rise_exit;
-  for (long k = 0; k < n; ++k)
-    if (lg_relax_feed(&s, k, e[k], tol)) return k;
-  return -1;
+  return feed_all(&s, e, n, tol);
 }
 long run(const double *e, long n, double tol) { return run_ex(e, n, tol, 1); }
 """
@@ -149,3 +154,13 @@ def test_rate_floor_after_extrapolation(rule):
     assert e[early] - einf > 2e-10
     s = rule(e, tol, 0, np.log(0.9994))
     assert e[s] - einf < tol
+
+
+@pytest.mark.parametrize("bad", [float("nan"), float("inf"), float("-inf")])
+@pytest.mark.parametrize("at", [0, 5, 16, 40])
+def test_non_finite_energy_is_divergence(rule, bad, at):
+    """A non-finite energy ends the relaxation as diverged (-1) at the step it appears, whether or
+    not that step is a checkpoint; before it the rule behaves as usual."""
+    e = 1.0 + 0.5 ** np.arange(64.0)
+    e[at] = bad
+    assert rule(e, 1e-12) == -2 - at
