@@ -32,91 +32,118 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def pass_timings(lib, LG, psi_dev, n, reps=10):
-    """Average duration of each pass kind of the fused GPE step, launched through the C-ABI on a
-    scratch copy of the state (tables hold unit factors: the timing does not depend on values)."""
-    N = n * n * n
-    dev = {}
+def classify_pass(LG, axis, nfft, flags):
+    """Name of a pass kind of the fused GPE step from its axis (0 outer, 2 contiguous), number of
+    transforms and flags."""
+    if axis == 2 and flags & LG.PF_NOSTORE:
+        return "z_inv_pre"
+    if axis == 2 and nfft == 2 and flags & LG.PF_MID_GPE:
+        return "z_gpe"
+    if axis == 2 and nfft == 2 and flags & LG.PF_MID_POT:
+        return "z_pot"
+    if axis == 1 and nfft == 1:
+        return "y_fft"
+    if axis == 0 and nfft == 2 and flags & LG.PF_MID_KIN:
+        return "x_kin"
+    return f"axis{axis}_nfft{nfft}_flags{flags:#x}"
 
-    def dalloc(nbytes):
-        p = C.c_void_p()
-        if lib.hgrid_malloc(C.byref(p), nbytes):
-            raise RuntimeError(lib.hgrid_last_error().decode())
-        return p
 
-    work = dalloc(N * 16)
-    lib.hgrid_memcpy_d2d(work, psi_dev, N * 16)
-    rho = dalloc(N * 8)
-    vext = dalloc(N * 8)
-    lib.hgrid_memcpy_d2d(vext, rho, N * 8)
-    ones = np.tile(np.array([1.0, 0.0]), n)
-    kt = dalloc(n * 16)
-    lib.hgrid_memcpy_h2d(kt, ones.ctypes.data, n * 16)
-    k2 = dalloc(n * 8)
-    zeros = np.zeros(n)
-    lib.hgrid_memcpy_h2d(k2, zeros.ctypes.data, n * 8)
-    tw = dalloc(n * 16)
-    m = np.arange(n)
-    twh = np.stack([np.cos(2 * np.pi * m / n), -np.sin(2 * np.pi * m / n)], axis=1).ravel()
-    lib.hgrid_memcpy_h2d(tw, np.ascontiguousarray(twh).ctypes.data, n * 16)
-    scale = dalloc(16)
-    lib.hgrid_memcpy_h2d(scale, np.array([1.0, 0.0]).ctypes.data, 16)
-    part = dalloc(8 * 5 * 65536)
+def pass_bytes_per_point(name, S):
+    """Algorithmic HBM bytes per grid point of one launch of each pass kind of the deferred GPE
+    step (S = bytes per stored complex value): the state read and written once, plus what the
+    fused pointwise work reads or writes (vext 8 B, V_eff 8 B)."""
+    return {"z_inv_pre": S + 8 + 8,  # S read (not stored), vext read, V_eff written
+            "y_fft": 2 * S, "x_kin": 2 * S,
+            "z_gpe": 2 * S + 8}.get(name)  # V_eff read
 
-    def mk(axis, **kw):
-        p = LG.hgrid_pass()
-        p.in_ = work
-        p.out = work
-        p.prec = LG.HGRID_C128
-        p.n0 = p.n1 = p.n2 = n
-        p.axis = axis
-        p.nfft = 1
-        p.dir1, p.dir2 = -1, +1
-        p.tw = tw
-        p.post = 1.0
-        p.mid = 1.0
-        p.partials = part
-        for k, v in kw.items():
-            setattr(p, k, v)
-        return p
 
-    kinds = {
-        # name: (pass, launches per step, algorithmic bytes per point per launch)
-        # the state stays transformed along z between steps: a read-only z pass gives V_eff and the
-        # start-of-step reductions, the y passes carry the pending scale and the output norm
-        "z_inv_pre": (mk(2, dir1=+1, flags=LG.PF_PRESCALE | LG.PF_PRE_VEFF | LG.PF_PRE_RED | LG.PF_PRE_AFTER |
-                         LG.PF_NOSTORE, scale=scale, rho_out=rho, vext=vext, g=0.1), 1, 32),
-        "y_fft": (mk(1, post=float(n)), 4, 32),
-        "x_kin": (mk(0, nfft=2, flags=LG.PF_MID_KIN | LG.PF_MID_KRED | LG.PF_KIN_REAL, ktab_a=kt, ktab_1=kt, ktab_2=kt,
-                     k2_a=k2, k2_1=k2, k2_2=k2), 2, 32),
-        "z_gpe": (mk(2, nfft=2, dir1=+1, dir2=-1, flags=LG.PF_MID_GPE, rho=rho, c_re=-0.05, c_im=0.0), 1, 40),
-    }
-    t = LG.HipTimer()
+def insitu_passes(lib, LG, step_fn, steps, N, S):
+    """Time every pass kernel of `steps` real steps with HIP events on the library stream
+    (grid_hip_pass_timing) and group the launches by pass kind: average ms per launch, launches
+    per step, algorithmic bytes per launch and their rate."""
+    cap = 4096
+    lib.grid_hip_pass_timing(1)
+    for _ in range(steps):
+        step_fn()
+    arr = {k: (C.c_int * cap)() for k in ("axis", "nfft", "prec")}
+    flags, lens, ms = (C.c_uint * cap)(), (C.c_long * cap)(), (C.c_double * cap)()
+    k = lib.grid_hip_pass_times(cap, arr["axis"], arr["nfft"], flags, lens, arr["prec"], ms)
+    lib.grid_hip_pass_timing(0)
+    groups = {}
+    for i in range(k):
+        groups.setdefault(classify_pass(LG, arr["axis"][i], arr["nfft"][i], flags[i]), []).append(ms[i])
     out = {}
-    # measured ceiling: a streaming copy of one grid (N * 16 B read + N * 16 B written)
-    for _ in range(2):
-        lib.hgrid_stream_copy(work, psi_dev, N * 16)
+    for name, t in groups.items():
+        avg = float(np.mean(t))
+        bpp = pass_bytes_per_point(name, S)
+        rec = dict(ms=avg, count=len(t) / steps, min_ms=float(np.min(t)), max_ms=float(np.max(t)))
+        if bpp is not None:
+            rec.update(bytes=bpp * N, gbps=bpp * N / (avg * 1e-3) / 1e9, frac=bpp * N / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS)
+        out[name] = rec
+    return out
+
+
+def copy_ceiling(lib, nbytes, reps=10):
+    """hgrid_stream_copy (16-byte loads and stores, eight per thread) of one buffer into another:
+    read + write bytes over the average time."""
+    a, b = C.c_void_p(), C.c_void_p()
+    if lib.hgrid_malloc(C.byref(a), nbytes) or lib.hgrid_malloc(C.byref(b), nbytes):
+        raise RuntimeError(lib.hgrid_last_error().decode())
+    lib.hgrid_memset(a, 0, nbytes)
+    for _ in range(3):
+        lib.hgrid_stream_copy(b, a, nbytes)
+    import libgrid_amd as LG
+    t = LG.HipTimer()
     t.start()
     for _ in range(reps):
-        lib.hgrid_stream_copy(work, psi_dev, N * 16)
+        lib.hgrid_stream_copy(b, a, nbytes)
     ms = t.stop() / reps
-    out["_copy"] = dict(ms=ms, count=0, bytes=32 * N, gbps=32 * N / (ms * 1e-3) / 1e9)
-    log(f"stream copy    {ms * 1e3:8.1f} us  {out['_copy']['gbps']:7.0f} GB/s  (measured ceiling)")
-    lib.hgrid_memcpy_d2d(work, psi_dev, N * 16)
-    for name, (p, count, bpp) in kinds.items():
-        for _ in range(2):
-            if lib.hgrid_fft_pass(C.byref(p)):
-                raise RuntimeError(f"pass {name} failed: {lib.hgrid_last_error().decode()}")
-        lib.hgrid_sync()
-        t.start()
-        for _ in range(reps):
-            lib.hgrid_fft_pass(C.byref(p))
-        ms = t.stop() / reps
-        out[name] = dict(ms=ms, count=count, bytes=bpp * N, gbps=bpp * N / (ms * 1e-3) / 1e9)
-        log(f"pass {name:11s} {ms * 1e3:8.1f} us  {out[name]['gbps']:7.0f} GB/s  x{count}/step")
     t.close()
-    for p in (work, rho, vext, kt, k2, tw, scale, part):
-        lib.hgrid_free(p)
+    lib.hgrid_free(a)
+    lib.hgrid_free(b)
+    gbps = 2 * nbytes / (ms * 1e-3) / 1e9
+    log(f"stream copy {nbytes >> 20} MiB: {ms * 1e3:.1f} us, {gbps:.0f} GB/s")
+    return dict(bytes_each_way=nbytes, ms=ms, GBps=gbps, frac=gbps / HBM_PEAK_GBPS)
+
+
+def fp64_512_bench(lib, LG, steps=10, warmup=3):
+    """The FP64 GPE imaginary-time step at 512^3 (spacing 0.125, the same trap and barrier, g 0.1,
+    dt 0.05, renormalised): a 2 GiB state, eight times the 256 MB Infinity Cache, so every pass
+    streams from HBM.  Steps/s, the step's fraction of the HBM roofline on the bytes it moves, and
+    every pass kind timed in place with HIP events."""
+    n, step, g, dt = 512, 0.125, 0.1, 0.05
+    w = lib.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"gpe512")
+    vr = lib.rgrid_alloc(n, n, n, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vext512")
+    x = (np.arange(n) - n // 2) * step
+    r2 = (x * x)[:, None, None] + (x * x)[None, :, None] + (x * x)[None, None, :]
+    LG.set_rgrid(vr, 0.5 * 0.01 * r2 + 0.05 * np.exp(-r2 / 4.0))
+    del r2
+    lib.grid_wf_gaussian_start(w, 10.0)
+    tstep = LG.c_complex.of(-1j * dt)
+
+    def one():
+        lib.grid_wf_gpe_propagate(w, vr, g, tstep, b"\x01")
+
+    for _ in range(warmup):
+        one()
+    lib.grid_hip_sync()
+    t = LG.HipTimer()
+    t.start()
+    for _ in range(steps):
+        one()
+    ms = t.stop() / steps
+    t.close()
+    N = n ** 3
+    passes = insitu_passes(lib, LG, one, 3, N, 16)
+    actual = sum(v["bytes"] * v["count"] for v in passes.values() if "bytes" in v)
+    lib.grid_wf_free(w)
+    lib.rgrid_free(vr)
+    out = dict(workload="gpe_barrier 512^3 complex128 imaginary-time step (spacing 0.125, dt 0.05, g 0.1, renormalised)",
+               ms_per_step=ms, steps_per_s=1000.0 / ms, actual_bytes_per_step=actual,
+               actual_GBps=actual / (ms * 1e-3) / 1e9, frac=actual / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, passes=passes)
+    log(f"fp64 512^3: {ms:.3f} ms/step, {actual / (ms * 1e-3) / 1e9:.0f} GB/s on {actual / N:.0f} B/pt "
+        f"({out['frac']:.3f} of 8 TB/s); passes " +
+        ", ".join(f"{k} {v['ms'] * 1e3:.0f} us {v.get('frac', 0):.2f}" for k, v in passes.items()))
     return out
 
 
@@ -186,7 +213,7 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
     return out
 
 
-def warm_start_bench(lib, count):
+def warm_start_bench(lib, count, subset=False):
     """Wall time to a 1e-10 Eh ground state on held-out 256^3 wells potentials: plain Gaussian
     start with FP64 throughout, against the trained network's start with FP32 relaxation
     (to 3e-8) then FP64 (to 1e-10): with vector extrapolation in both stages (net32ee, the
@@ -198,8 +225,8 @@ def warm_start_bench(lib, count):
         log(f"warm start: no trained network at {WS.WEIGHTS}; skipped")
         return None
     net = lib.grid_surrogate_load(WS.WEIGHTS.encode(), 1)
-    modes = ("gauss64", "net32ee", "net32e", "net32", "net64")
-    recs = WS.evaluate(lib, net, count, WS.FINE, modes)
+    modes = ("gauss64", "net32ee", "net32e", "net32", "net64") if not subset else ("gauss64", "net32ee")
+    recs = WS.evaluate(lib, net, count, WS.FINE, modes, indices=WS.HELDOUT_SUBSET if subset else None)
     lib.grid_surrogate_free(net)
     summ = WS.summarize(recs, modes)
     log(f"warm start: {json.dumps(summ)}")
@@ -410,10 +437,12 @@ def main():
     ap.add_argument("--no-gs", action="store_true", help="skip the time-to-ground-state runs")
     ap.add_argument("--gs-max-steps", type=int, default=20000)
     ap.add_argument("--no-rows", action="store_true", help="skip the FP32 512^3, batched 2-D and propagator-variant measurements")
-    ap.add_argument("--rows", default="fp32_512,batch2d,propagators",
+    ap.add_argument("--rows", default="fp64_512,fp32_512,batch2d,propagators",
                     help="comma-separated subset of the other rows to measure")
-    ap.add_argument("--ws-count", type=int, default=2,
-                    help="held-out 256^3 wells potentials for the warm-start time-to-ground-state comparison")
+    ap.add_argument("--ws-count", type=int, default=-1,
+                    help="held-out 256^3 wells potentials for the warm-start time-to-ground-state comparison: "
+                         "the first N in all five modes, or -1 (default) for the fixed subset of every 8th "
+                         "of the 200 (25 potentials) in the plain and fast modes, 0 to skip")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -451,20 +480,17 @@ def main():
     energy = lib.grid_wf_gpe_last_energy(w)
     log(f"{args.steps} steps: {ms_step:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step, E = {energy:.12f}")
 
-    # per-kernel timing on the psi device array (dev->d is the first field of grid_dev after kind, prec, n)
-    class _dev(C.Structure):
-        _fields_ = [("kind", C.c_int), ("prec", C.c_int), ("n", C.c_size_t), ("d", C.c_void_p)]
-
-    psi_dev = C.cast(w.contents.grid.contents.dev, C.POINTER(_dev)).contents.d
-    kinds = pass_timings(lib, LG, C.c_void_p(psi_dev), n)
-    copy = kinds.pop("_copy")
-    dom = max(kinds.items(), key=lambda kv: kv[1]["ms"] * kv[1]["count"])
-    dname, d = dom
-    sum_ms = sum(v["ms"] * v["count"] for v in kinds.values())
+    # every pass kernel of further real steps, timed in place with HIP events on the library stream
     N = n**3
+    kinds = insitu_passes(lib, LG, lambda: lib.grid_wf_gpe_propagate(w, vr, g, tstep, b"\x01"), args.steps, N, 16)
+    dname, d = max(((k, v) for k, v in kinds.items() if "bytes" in v), key=lambda kv: kv[1]["ms"] * kv[1]["count"])
+    sum_ms = sum(v["ms"] * v["count"] for v in kinds.values())
+    actual_bytes = sum(v["bytes"] * v["count"] for v in kinds.values() if "bytes" in v)
     model_bytes = 400 * N  # (4 FFTs x 3 sweeps x 2 x 16 B + 16 B potential) per point
-    actual_bytes = sum(v["bytes"] * v["count"] for v in kinds.values())
-
+    for k, v in kinds.items():
+        log(f"pass {k:10s} {v['ms'] * 1e3:8.1f} us x{v['count']:.0f}/step" +
+            (f"  {v['gbps']:6.0f} GB/s ({v['frac']:.3f})" if "gbps" in v else ""))
+    copies = {"256MiB": copy_ceiling(lib, N * 16), "2GiB": copy_ceiling(lib, 8 * N * 16)}
     traffic = profiled_traffic(dname, n)
 
     gs = None
@@ -472,8 +498,8 @@ def main():
         gs = ground_state(lib, LG, psi, vext, prm, n, args.gs_max_steps)
 
     ws = None
-    if not args.no_gs and args.ws_count > 0:
-        ws = warm_start_bench(lib, args.ws_count)
+    if not args.no_gs and args.ws_count != 0:
+        ws = warm_start_bench(lib, args.ws_count, subset=args.ws_count < 0)
 
     rows = {}
     if not args.no_rows:
@@ -484,6 +510,8 @@ def main():
             rows["batch2d"] = batch2d_bench(lib, LG)
         if "propagators" in which:
             rows["propagators"] = propagators_bench(lib, LG)
+        if "fp64_512" in which:
+            rows["fp64_512"] = fp64_512_bench(lib, LG)
 
     cpu = None
     if not args.no_cpu:
@@ -510,13 +538,17 @@ def main():
         "roofline": {"bound": "hbm", "kernel": f"fft_pass {dname}", "achieved": d["gbps"], "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": d["gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]},
-        "step_roofline": {"model_bytes_per_step": model_bytes, "achieved_GBps": model_bytes / (ms_step * 1e-3) / 1e9,
-                          "frac": model_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-                          "actual_bytes_per_step": actual_bytes,
-                          "actual_GBps": actual_bytes / (ms_step * 1e-3) / 1e9,
-                          "sum_of_pass_ms": sum_ms},
-        "pass_ms": {k: v["ms"] for k, v in kinds.items()},
-        "copy_ceiling_GBps": copy["gbps"],
+        # the step on the bytes it moves (264 B/pt: the deferred z transform cancels two z sweeps
+        # per step); the 400 B/pt model of a 12-sweep step is kept for reference only
+        "step_roofline": {"actual_bytes_per_step": actual_bytes,
+                          "achieved_GBps": actual_bytes / (ms_step * 1e-3) / 1e9,
+                          "frac": actual_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                          "model_bytes_per_step_12_sweeps": model_bytes,
+                          "sum_of_pass_ms": sum_ms,
+                          "note": "the 256 MiB state fits the 256 MB Infinity Cache, so part of the traffic is served "
+                                  "from it; other_rows.fp64_512 is the same step at 512^3 (2 GiB), streaming from HBM"},
+        "passes": kinds,
+        "copy_ceiling": copies,
         "energy": energy,
         "time_to_ground_state": gs,
         "warm_start": ws,

@@ -299,6 +299,14 @@ INT grid_wf_gpe_relax_extrapolated(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL t
 /* Relaxation replays batches of 16 steps as a HIP graph after the first batch (default on);
  * grid_hip_graphs(0) launches every step eagerly instead. */
 void grid_hip_graphs(int enable);
+/* Per-pass timing (this build's addition; plays the role of the reference's cuda_statistics,
+ * src/cuda.c): grid_hip_pass_timing(1) starts recording every FFT/propagation pass kernel
+ * between two HIP events on the library stream (up to 4096 passes; enabling again resets the
+ * record), 0 stops it.  grid_hip_pass_times copies up to max records: the pass's axis (0 outer,
+ * 2 contiguous), number of transforms, flags (HGRID_PF_*), length, precision and its time in
+ * milliseconds; it waits for those kernels.  Not for use under graph capture. */
+void grid_hip_pass_timing(int enable);
+INT grid_hip_pass_times(INT max, int *axis, int *nfft, unsigned *flags, long *len, int *prec, double *ms);
 
 /* ---- batched 2-D ground states (this build's addition) ----------------------------------
  * Relax nsolves independent 1 x n x n wavefunctions (n in {64, 96, 128, 192, 256}) in

@@ -293,6 +293,9 @@ PROTOTYPES = {
     "grid_batch2d_solve_times": (L, [C.POINTER(D), C.POINTER(D)]),
     "grid_batch2d_set_straggler_cap": (V, [L]),
     "grid_hip_graphs": (V, [C.c_int]),
+    "grid_hip_pass_timing": (V, [C.c_int]),
+    "grid_hip_pass_times": (L, [L, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint), C.POINTER(L),
+                                C.POINTER(C.c_int), C.POINTER(D)]),
     "grid_batch2d_set_straggler_extrapolation": (V, [L]),
     "grid_batch2d_last_phase2": (V, [C.POINTER(L), C.POINTER(D)]),
     "grid_surrogate_alloc": (C.c_void_p, [L, L, D, L]),

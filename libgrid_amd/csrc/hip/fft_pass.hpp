@@ -42,32 +42,113 @@ enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3 };
 
 
 // Threads per pencil (TPP) and values per thread (R) for each supported length L = R * TPP:
-// powers of two up to 1024 and 3 * 2^k up to 384.  R must be a multiple of TPP and divide 192
-// (the in-register DFT sizes).
+// powers of two up to 1024 and 3 * 2^k up to 384.  Two register stages (R-point DFTs, then
+// R / TPP TPP-point DFTs per thread) need R to be a multiple of TPP; L = 512 instead runs three
+// 8-point stages with 64 threads per pencil (THREE), which keeps 8 values per thread instead of
+// 32 (the two-stage 16 x 32 split held 32 values through two transforms and ran at 1-2
+// waves/SIMD).
 template <int L> struct Split {
-  static constexpr int TPP = (L <= 16 || L == 24) ? 1
+  static constexpr bool THREE = L == 512;
+  static constexpr int TPP = THREE ? 64
+                             : (L <= 16 || L == 24) ? 1
                              : L == 32 || L == 48 || L == 96 ? 4
                              : L == 64 || L == 128 || L == 192 || L == 384 ? 8
-                             : L == 256 || L == 512 ? 16
+                             : L == 256 ? 16
                              : 32;
   static constexpr int R = L / TPP;
   static_assert(R * TPP == L, "unsupported length");
-  static_assert(R % TPP == 0, "stage 2 needs R to be a multiple of TPP");
+  static_assert(THREE || R % TPP == 0, "stage 2 needs R to be a multiple of TPP");
+  static_assert(!THREE || (R == 8 && TPP == 64), "the three-stage split is 8 x 8 x 8");
 };
 
 #define LG_FOR_EACH_LENGTH(X) \
   X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
 
-template <typename C, int L> struct Geom {
+template <typename C, int L, bool CONTIG> struct Geom {
   static constexpr int TPP = Split<L>::TPP;
   static constexpr int R = Split<L>::R;
-  // workgroup size: 256 threads unless one real half of the LDS exchange exceeds 32 KiB
-  static constexpr int NT = (TPP > 1 && (long)R * (long)sizeof(C) * 128 > 32768) ? 128 : 256;
+  static constexpr bool THREE = Split<L>::THREE;
+  // workgroup size: two-stage lengths take 256 threads unless one real half of the LDS exchange
+  // exceeds 32 KiB.  Three-stage: contiguous pencils are one wave each, four per workgroup;
+  // strided workgroups take the pencils of 128 consecutive bytes of a row (16 FP32 / 8 FP64).
+  static constexpr int NT = THREE ? (CONTIG ? 256 : TPP * (int)(128 / sizeof(C)))
+                                  : (TPP > 1 && (long)R * (long)sizeof(C) * 128 > 32768) ? 128 : 256;
   static constexpr int P = NT / TPP;  // pencils per workgroup
-  // the exchange moves real parts, then imaginary parts, through one real-valued buffer
-  static constexpr int LDS_CONTIG = TPP > 1 ? P * R * (TPP + 1) : 1;
-  static constexpr int LDS_STRIDED = TPP > 1 ? P * R * TPP : 1;
+  // complex entries per pencil in the three-stage exchange buffer (512 + 4: consecutive pencils
+  // start 4 entries apart in the LDS banks)
+  static constexpr int PS3 = 516;
+  // two-stage: the exchange moves real parts, then imaginary parts, through one real buffer;
+  // three-stage: interleaved complex entries (LDS sizes are counted in reals)
+  static constexpr int LDS_CONTIG = THREE ? 2 * P * PS3 : TPP > 1 ? P * R * (TPP + 1) : 1;
+  static constexpr int LDS_STRIDED = THREE ? 2 * P * PS3 : TPP > 1 ? P * R * TPP : 1;
 };
+
+// Positions in a pencil's exchange buffer of the three-stage transform (complex entries).
+// Exchange 1 moves Y[t1 + 8 t2][k1]: written by thread t1 + 8 t2 (k1 = 0..7), read by thread
+// t1 + 8 k1 (t2 = 0..7); exchange 2 moves Z[t1, k1][k2]: written by thread t1 + 8 k1 (k2 = 0..7),
+// read by thread k1 + 8 k2 (t1 = 0..7).  The XOR terms permute entries within rows of 8 so that
+// the 64 lanes of a wave access spread over the LDS banks for every one of these patterns.
+__device__ __forceinline__ int x1_index(int k1, int t2, int t1) { return (k1 << 6) + ((t2 ^ (k1 & 1)) << 3) + t1; }
+__device__ __forceinline__ int x2_index(int k1, int k2, int t1) {
+  return (k2 << 6) + (k1 << 3) + (t1 ^ ((k2 + (k1 >> 1)) & 7));
+}
+
+// Ordering of a pencil's LDS exchange.  A contiguous three-stage pencil is exactly one wave,
+// whose LDS accesses execute in order: only the compiler must be kept from moving them across
+// this point.  Otherwise the pencil spans the waves of the workgroup.
+template <bool WAVE> __device__ __forceinline__ void pencil_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// One 512-point transform of the pencil owned by (q, t), 64 threads of 8 values, thread t holding
+// x[t + 64 j].  With n = t1 + 8 t2 + 64 j and k = k1 + 8 k2 + 64 k3:
+//   stage 1: 8-point DFTs over j, twiddle W_512^{t k1}        (thread t = t1 + 8 t2)
+//   stage 2: 8-point DFTs over t2, twiddle W_64^{t1 k2}       (thread t1 + 8 k1)
+//   stage 3: 8-point DFTs over t1                             (thread k1 + 8 k2)
+// so thread t ends holding X[t + 64 k3]: the input distribution.  Both twiddles come from the
+// length-512 table (W_64^m = W_512^{8m}).
+template <typename C, bool CONTIG>
+__device__ __forceinline__ void fft_core3(C (&v)[8], int t, int q, bool inverse, const C *__restrict__ tw,
+                                          typename real_of<C>::type *ldsr) {
+  constexpr int PS = Geom<C, 512, CONTIG>::PS3;
+  C *lds = reinterpret_cast<C *>(ldsr) + q * PS;
+  if (inverse) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = cconj(v[j]);
+  }
+  const C *twl = tw;
+  __asm__ volatile("" : "+s"(twl));
+  dft<8>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < 8; ++k1) v[k1] = cmul(v[k1], twl[t * k1]);
+  pencil_sync<CONTIG>();  // the previous transform's exchange 2 may still be reading
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) lds[x1_index(k1, t >> 3, t & 7)] = v[k1];
+  pencil_sync<CONTIG>();
+  const int a = t & 7, b = t >> 3;  // stage 2: (t1, k1) = (a, b); stage 3: (k1, k2) = (a, b)
+#pragma unroll
+  for (int t2 = 0; t2 < 8; ++t2) v[t2] = lds[x1_index(b, t2, a)];
+  dft<8>(v);
+#pragma unroll
+  for (int k2 = 1; k2 < 8; ++k2) v[k2] = cmul(v[k2], twl[8 * a * k2]);
+  pencil_sync<CONTIG>();  // exchange 1's reads of this buffer are done
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) lds[x2_index(b, k2, a)] = v[k2];
+  pencil_sync<CONTIG>();
+#pragma unroll
+  for (int t1 = 0; t1 < 8; ++t1) v[t1] = lds[x2_index(a, b, t1)];
+  dft<8>(v);
+  if (inverse) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = cconj(v[j]);
+  }
+}
 
 template <bool CONTIG, int R, int TPP, int P>
 __device__ __forceinline__ int lds_index(int q, int k1, int t) {
@@ -75,12 +156,11 @@ __device__ __forceinline__ int lds_index(int q, int k1, int t) {
   else return (k1 * TPP + t) * P + q;
 }
 
-// ---- one L-point transform of the pencil owned by (q, t) ----------------------------------
-
+// One L-point transform in two register stages (the decomposition at the top of this file).
 template <typename C, int L, bool CONTIG>
-__device__ __forceinline__ void fft_core(C (&v)[Split<L>::R], int t, int q, bool inverse, const C *__restrict__ tw,
-                                         typename real_of<C>::type *lds) {
-  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, P = Geom<C, L>::P;
+__device__ __forceinline__ void fft_core2(C (&v)[Split<L>::R], int t, int q, bool inverse, const C *__restrict__ tw,
+                                          typename real_of<C>::type *lds) {
+  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, P = Geom<C, L, CONTIG>::P;
   if (inverse) {
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = cconj(v[j]);
@@ -125,6 +205,19 @@ __device__ __forceinline__ void fft_core(C (&v)[Split<L>::R], int t, int q, bool
   }
 }
 
+// ---- one L-point transform of the pencil owned by (q, t) ----------------------------------
+
+template <typename C, int L, bool CONTIG>
+__device__ __forceinline__ void fft_core(C (&v)[Split<L>::R], int t, int q, bool inverse, const C *__restrict__ tw,
+                                         typename real_of<C>::type *lds) {
+  if constexpr (Split<L>::THREE) {
+    fft_core3<C, CONTIG>(v, t, q, inverse, tw, lds);
+  } else {
+    fft_core2<C, L, CONTIG>(v, t, q, inverse, tw, lds);
+  }
+}
+
+
 // exp(c * V) with complex c and complex V (FP64 argument).
 template <typename C> __device__ __forceinline__ C phase_factor(double cr, double ci, double vr, double vi);
 
@@ -156,12 +249,13 @@ template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, do
 // C is the arithmetic type, S the storage type of psi (and of a complex potential): S = C, or
 // S = float2 with C = double2 ("FP32 storage, FP64 arithmetic": only the stores round).
 template <typename C, int L, bool CONTIG, int KIND, typename S = C>
-__global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_pass a) {
+__global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(const hgrid_pass a) {
   using Rl = typename real_of<C>::type;
-  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, NT = Geom<C, L>::NT, P = Geom<C, L>::P;
-  constexpr int LDS_N = CONTIG ? Geom<C, L>::LDS_CONTIG : Geom<C, L>::LDS_STRIDED;
+  using G = Geom<C, L, CONTIG>;
+  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, NT = G::NT, P = G::P;
+  constexpr int LDS_N = CONTIG ? G::LDS_CONTIG : G::LDS_STRIDED;
   constexpr unsigned CS = sizeof(S);  // bytes per stored element
-  __shared__ Rl lds[LDS_N];
+  __shared__ __align__(16) Rl lds[LDS_N];
   __shared__ double red_lds[NT / 64][HGRID_RED_NCHAN];
 
   const int tid = threadIdx.x;
@@ -393,7 +487,7 @@ __global__ void __launch_bounds__((Geom<C, L>::NT)) fft_pass_kernel(const hgrid_
 }
 
 template <typename C, int L, bool CONTIG, int KIND, typename S> int launch(const hgrid_pass *p) {
-  constexpr int NT = Geom<C, L>::NT, P = Geom<C, L>::P;
+  constexpr int NT = Geom<C, L, CONTIG>::NT, P = Geom<C, L, CONTIG>::P;
   const long npencils = (p->n0 * p->n1 * p->n2) / L;
   const long nblocks = (npencils + P - 1) / P;
   if (nblocks <= 0) return 0;
@@ -422,9 +516,9 @@ template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long 
   }
 }
 
-template <typename C> long pencils_per_block(long L) {
+template <typename C, bool CONTIG> long pencils_per_block(long L) {
 #define LG_CASE(n) \
-  case n: return Geom<C, n>::P;
+  case n: return Geom<C, n, CONTIG>::P;
   switch (L) {
     LG_FOR_EACH_LENGTH(LG_CASE)
     default: return 0;

@@ -48,10 +48,60 @@ void lg_pass_init(hgrid_pass *p, const lg_shape *s, int prec, void *in, void *ou
   p->tw = lg_twiddles(s->n[axis], prec == HGRID_C64 ? HGRID_C64 : HGRID_C128);
 }
 
+/* ---- per-pass timing ------------------------------------------------------------------------
+ * While enabled, every pass kernel is bracketed by two HIP events on the library stream and its
+ * description kept, so a caller can attribute the time of a real step to its passes (the
+ * counterpart of the reference's cuda_statistics, src/cuda.c). */
+#define LG_PASS_TRACE_MAX 4096
+typedef struct {
+  void *ev[2];
+  int axis, nfft, prec;
+  unsigned flags;
+  long len;
+} pass_record;
+static pass_record g_trace[LG_PASS_TRACE_MAX];
+static int g_trace_on = 0, g_trace_n = 0, g_trace_events = 0;
+
+void grid_hip_pass_timing(int enable) {
+  if (enable && !g_trace_events) {
+    lg_ensure_device();
+    for (int i = 0; i < LG_PASS_TRACE_MAX; ++i)
+      for (int k = 0; k < 2; ++k) lg_hip(hgrid_event_create(&g_trace[i].ev[k]), "hipEventCreate (pass timing)");
+    g_trace_events = 1;
+  }
+  g_trace_on = enable ? 1 : 0;
+  g_trace_n = 0;
+}
+
+INT grid_hip_pass_times(INT max, int *axis, int *nfft, unsigned *flags, long *len, int *prec, double *ms) {
+  const INT n = g_trace_n < max ? g_trace_n : max;
+  for (INT i = 0; i < n; ++i) {
+    float t = 0.0f;
+    lg_hip(hgrid_event_elapsed_ms(g_trace[i].ev[0], g_trace[i].ev[1], &t), "hipEventElapsedTime (pass timing)");
+    axis[i] = g_trace[i].axis;
+    nfft[i] = g_trace[i].nfft;
+    flags[i] = g_trace[i].flags;
+    len[i] = g_trace[i].len;
+    prec[i] = g_trace[i].prec;
+    ms[i] = t;
+  }
+  return n;
+}
+
 void lg_run_pass(hgrid_pass *p) {
+  pass_record *r = (g_trace_on && g_trace_n < LG_PASS_TRACE_MAX) ? &g_trace[g_trace_n++] : NULL;
+  if (r) lg_hip(hgrid_event_record(r->ev[0]), "hipEventRecord (pass timing)");
   int rc = hgrid_fft_pass(p);
   if (rc) lg_fatal("FFT pass (axis %d, shape %ld x %ld x %ld) failed: %s", p->axis, p->n0, p->n1, p->n2,
                    hgrid_last_error()[0] ? hgrid_last_error() : "unsupported configuration");
+  if (r) {
+    lg_hip(hgrid_event_record(r->ev[1]), "hipEventRecord (pass timing)");
+    r->axis = p->axis;
+    r->nfft = p->nfft;
+    r->flags = p->flags;
+    r->len = p->axis == 0 ? p->n0 : p->axis == 1 ? p->n1 : p->n2;
+    r->prec = p->prec;
+  }
 }
 
 static void copy_scaled(void *in, void *out, int prec, long n, double post, const double *scale) {

@@ -184,13 +184,20 @@ MODES = {  # name: (use the network, FP32-storage stage tolerance or None, its a
 }
 
 
+# A fixed subset of the 200 held-out potentials, every 8th (#0, #8, ..., #192): the same draws
+# of the seed's stream whatever the caller, spread over the whole set
+HELDOUT_SUBSET = tuple(range(0, 200, 8))
+
+
 def evaluate(lg, net, count, n=FINE, modes=("gauss64", "net32", "net64"), seed=HELDOUT_SEED, first=0,
-             max_steps=200000, tol=1e-10, every=1):
-    """Relax held-out potentials first .. first+count-1 of the seed's stream on n^3 from each
-    start; returns per-potential records.  The converged energies of all modes must agree."""
-    prm = W.wells_3d_params(seed, first + count, BOX)
+             max_steps=200000, tol=1e-10, every=1, indices=None):
+    """Relax held-out potentials first .. first+count-1 of the seed's stream (or the given
+    indices) on n^3 from each start; returns per-potential records.  The converged energies of
+    all modes must agree."""
+    idx = list(indices) if indices is not None else list(range(first, first + count))
+    prm = W.wells_3d_params(seed, max(idx) + 1, BOX)
     out = []
-    for i in range(first, first + count):
+    for i in idx:
         v = W.wells_3d(prm[i], n, BOX / n)
         p = Problem(lg, n, v)
         rec = dict(index=i, wells=len(prm[i]["depth"]))

@@ -56,7 +56,9 @@ def rand_c(shape, seed):
 # powers of two up to 1024 and 3 * 2^k up to 384 on every axis, in 1-, 2- and 3-D shapes
 FFT_SHAPES = [(16, 16, 16), (8, 4, 2), (4, 32, 8), (2, 2, 256), (1, 1, 64), (64, 64, 64), (32, 64, 128), (512, 4, 2),
               (2, 1024, 2), (2, 16, 1), (1, 1, 1), (128, 1, 1), (1, 256, 256),
-              (48, 96, 192), (1, 192, 192), (2, 384, 2), (3, 6, 12), (24, 1, 96), (384, 2, 3), (96, 64, 48)]
+              (48, 96, 192), (1, 192, 192), (2, 384, 2), (3, 6, 12), (24, 1, 96), (384, 2, 3), (96, 64, 48),
+              # 512-point axes (three register stages): contiguous, strided, partial workgroups
+              (3, 5, 512), (3, 512, 5), (512, 3, 7), (16, 512, 32), (512, 2, 512)]
 
 
 def gpu_length(n):
@@ -94,6 +96,24 @@ def test_fft_golden_vectors(lg):
         lg.cgrid_inverse_fft(g)
         assert rel_l2(LG.get_cgrid(g), gold["bwd_" + tag]) < TOL, tag
         lg.cgrid_free(g)
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (3, 5, 512), (3, 512, 5), (512, 3, 7), (16, 512, 32), (1, 192, 96)])
+def test_fft_single_precision(lg, orc, shape):
+    """complex64 storage and FP32 butterflies with the FP64-accurate twiddle table rounded once:
+    relative L2 against the FP64 oracle of the FP32-rounded input stays at FP32 rounding."""
+    case(f"fft fp32 {shape}")
+    a = rand_c(shape, 19).astype(np.complex64).astype(np.complex128)
+    g = new_cgrid(lg, a)
+    lg.cgrid_set_precision(g, b"\x01")
+    lg.cgrid_fft(g)
+    fa = LG.get_cgrid(g)
+    err = rel_l2(fa, orc.fft3d(a, -1))
+    lg.cgrid_inverse_fft(g)
+    err2 = rel_l2(LG.get_cgrid(g), a * a.size)
+    print(f"fp32 fft {shape}: forward {err:.2e}, round trip {err2:.2e}")
+    assert err < 2e-6 and err2 < 4e-6
+    lg.cgrid_free(g)
 
 
 def test_fft_length_set_matches_library(lg):
