@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 import tempfile
 
 import pytest
@@ -169,3 +170,22 @@ def test_unsupported_propagator_combinations_abort(propagator, setup, message):
                        env=dict(os.environ, HIP_VISIBLE_DEVICES="", PYTHONPATH=ROOT))
     assert p.returncode != 0
     assert "libgrid:" in p.stderr and message in p.stderr
+
+
+def test_gpu_fft_test_shapes_are_in_the_transform_set():
+    """Every shape the GPU FFT parity test runs has axis lengths the library transforms (a length
+    outside the set aborts on the host before any GPU work, which would end the whole GPU test
+    process), and lengths outside the set are refused with the library's message."""
+    from test_gpu_parity import FFT32_SHAPES, FFT_SHAPES
+
+    lib = libgrid_amd.load()
+    for shape in FFT_SHAPES + FFT32_SHAPES:
+        for n in shape:
+            assert n == 1 or lib.hgrid_fft_supported(n), (shape, n)
+    code = ("import libgrid_amd as LG; lib = LG.load(); "
+            "g = lib.cgrid_alloc(3, 5, 512, 0.2, LG.boundary('cgrid_value_outside_periodic'), None, b'g'); "
+            "lib.cgrid_fft(g)")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode != 0
+    assert "libgrid: FFT length 5 is not supported" in r.stderr

@@ -76,13 +76,16 @@ def relax_extrapolated(lg, shape, step, psi, v, prec, graphs, stride=64, tol=1e-
     return e.value, steps, nx_.value, out
 
 
-@pytest.mark.parametrize("shape,prec", [((1, 128, 128), b"\x01"), ((1, 192, 192), b"\x01"), ((32, 32, 32), b"\x00")])
+@pytest.mark.parametrize("shape,prec", [((1, 128, 128), b"\x01"), ((1, 192, 192), b"\x01"), ((32, 64, 32), b"\x00")])
 def test_graph_replay_equals_eager_extrapolated(lg, shape, prec):
     """The extrapolated relaxation (snapshots, reductions and extrapolations between replayed
     batches; the straggler phase of the batched solver runs it on 2-D FP32 grids) gives the same
     steps, extrapolations, energy and state with graphs as without."""
     case(f"extrapolated relaxation graph replay vs eager {shape} prec {prec[0]}")
-    step = 0.2 if shape[0] == 1 else 0.3
+    # 3-D: a 16-bohr y axis keeps the wells 6 bohr apart both ways round the periodic box (a
+    # numpy relaxation of this case extrapolates first after 320 steps; at 32^3 and 0.3 bohr
+    # the wells couple across the boundary and the tail is too fast for the rule to fire)
+    step = 0.2 if shape[0] == 1 else 0.25
     x = [W.axes(n, step) for n in shape]
     r2 = x[0][:, None, None] ** 2 + x[1][None, :, None] ** 2 + x[2][None, None, :] ** 2
     # two nearly degenerate wells: a slow imaginary-time tail that the extrapolation shortens

@@ -57,8 +57,9 @@ def rand_c(shape, seed):
 FFT_SHAPES = [(16, 16, 16), (8, 4, 2), (4, 32, 8), (2, 2, 256), (1, 1, 64), (64, 64, 64), (32, 64, 128), (512, 4, 2),
               (2, 1024, 2), (2, 16, 1), (1, 1, 1), (128, 1, 1), (1, 256, 256),
               (48, 96, 192), (1, 192, 192), (2, 384, 2), (3, 6, 12), (24, 1, 96), (384, 2, 3), (96, 64, 48),
-              # 512-point axes (three register stages): contiguous, strided, partial workgroups
-              (3, 5, 512), (3, 512, 5), (512, 3, 7), (16, 512, 32), (512, 2, 512)]
+              # 512-point axes (three register stages): contiguous, strided, partial workgroups (18
+              # and 36 pencils against 4 or 8 per workgroup)
+              (3, 6, 512), (3, 512, 6), (512, 3, 12), (16, 512, 32), (512, 2, 512)]
 
 
 def gpu_length(n):
@@ -98,7 +99,10 @@ def test_fft_golden_vectors(lg):
         lg.cgrid_free(g)
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (3, 5, 512), (3, 512, 5), (512, 3, 7), (16, 512, 32), (1, 192, 96)])
+FFT32_SHAPES = [(16, 16, 16), (3, 6, 512), (3, 512, 6), (512, 3, 12), (16, 512, 32), (1, 192, 96)]
+
+
+@pytest.mark.parametrize("shape", FFT32_SHAPES)
 def test_fft_single_precision(lg, orc, shape):
     """complex64 storage and FP32 butterflies with the FP64-accurate twiddle table rounded once:
     relative L2 against the FP64 oracle of the FP32-rounded input stays at FP32 rounding."""
