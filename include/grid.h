@@ -311,18 +311,24 @@ INT grid_hip_pass_times(INT max, int *axis, int *nfft, unsigned *flags, long *le
 /* ---- batched 2-D ground states (this build's addition) ----------------------------------
  * Relax nsolves independent 1 x n x n wavefunctions (n in {64, 96, 128, 192, 256}) in
  * imaginary time with their potentials (rgrid, same shape) in one persistent GPU launch:
- * complex64 storage, FP64 twiddles / tables / reductions.  Each solve runs 2nd-order FFT
- * steps of length dt, renormalised every step, until its FFT energy is converged to tol by the
- * rule of grid_wf_gpe_relax (or max_steps).  The normalised states are written back to gwf[i]->grid; energy[i]
- * and steps[i] (may be NULL) receive the final energy and the step at which tol was met
- * (max_steps if never).  Returns the GPU time of the launch plus the wall time of the
- * straggler phase (grid_batch2d_set_straggler_cap) in seconds.  The first call on a
+ * complex64 storage, FP64 twiddle arguments / tables / reductions.  Each solve runs 2nd-order
+ * FFT steps of length dt, renormalised every step, until its FFT energy is converged to tol by
+ * the rule of grid_wf_gpe_relax (or max_steps), with the vector extrapolation of
+ * grid_wf_gpe_relax_extrapolated inside the launch.  The normalised states are written back
+ * to gwf[i]->grid; energy[i] and steps[i] (may be NULL) receive the energy of that state and
+ * the step at which tol was met (max_steps if never).  Returns the GPU time of the launch
+ * (plus the wall time of the optional straggler phase) in seconds.  The first call on a
  * device runs grid_batch2d_autotune unless a tuned tile is cached. */
 REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, REAL tol, INT max_steps, REAL *energy,
                            INT *steps);
-/* Solves still unconverged after this many steps of the persistent launch (default 2000; 0 =
- * no cap) are continued one at a time on the multi-kernel FP32 path, which spreads one grid over
- * the whole GPU; grid_batch2d_last_phase2 reports how many and their wall time. */
+/* Vector-extrapolation stride of the persistent launch in steps (default 64; 0 relaxes plainly). */
+void grid_batch2d_set_extrapolation(INT stride);
+/* Extrapolations the last grid_wf_batch2d_relax launch made, summed over its solves. */
+INT grid_batch2d_last_extrapolations(void);
+/* Optional straggler phase (off by default, 0): solves still unconverged after this many steps
+ * of the persistent launch are continued one at a time on the multi-kernel FP32 path, which
+ * spreads one grid over the whole GPU; grid_batch2d_last_phase2 reports how many and their
+ * wall time. */
 void grid_batch2d_set_straggler_cap(INT steps);
 /* The straggler phase relaxes with vector extrapolation every stride steps
  * (grid_wf_gpe_relax_extrapolated; default 64); 0 relaxes the stragglers plainly. */
@@ -336,8 +342,9 @@ const char *grid_batch2d_autotune(INT nsolves, wf **gwf, rgrid **potential, REAL
  * grid_wf_batch2d_relax call, in its input order; returns the count.  Arrays may be NULL. */
 INT grid_batch2d_solve_times(REAL *start, REAL *end);
 /* Record the energy entering each of the first len steps of every solve in later
- * grid_wf_batch2d_relax calls (0 turns it off); grid_batch2d_history copies solve's record of
- * the last call (at most len values, entries past its last step undefined) and returns the
+ * grid_wf_batch2d_relax calls (0 turns it off; NaN on steps between the stopping rule's
+ * checkpoints, where the launch forms no energy); grid_batch2d_history copies solve's record
+ * of the last call (at most len values, entries past its last step undefined) and returns the
  * number copied. */
 void grid_batch2d_record_history(INT len);
 INT grid_batch2d_history(INT solve, REAL *energy, INT len);

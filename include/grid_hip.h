@@ -224,28 +224,35 @@ typedef struct hgrid_b2d_job {
   void *psi;           /* n*n float2, row-major [y][x]; in: start state, out: normalised ground state */
   const float *pv;     /* n*n exp(-dt V) */
   const double *v;     /* n*n V (potential energy) */
-  const float *kt;     /* n: exp(-(dt/2) hbar k_m^2 / 2m) / n, the same table for both axes */
+  const float *kh;     /* n: exp(-(dt/2) hbar k_m^2 / 2m), the same table for both axes */
+  const float *kf;     /* n: exp(-dt hbar k_m^2 / 2m) */
   const double *k2;    /* n: k_m^2 */
-  const void *tw;      /* n: double2 exp(-2 pi i m / n) */
+  const void *tw;      /* n: float2 exp(-2 pi i m / n), rounded once */
+  void *snap, *diff;   /* n*n float2 each: vector-extrapolation snapshot and difference (stride > 0) */
   double dA;           /* step^2 */
-  double tfac;         /* hbar^2/(2m) * step^2 / n^2 */
+  double tfac;         /* hbar^2/(2m) * n^2 (energy = (tfac sum k^2|psi_k|^2 + sum V|psi|^2) / sum |psi|^2
+                          with the unnormalised transforms of the kernel) */
   double tol;          /* energy tolerance of the stopping rule (see grid_wf_gpe_relax) */
   int n, max_steps;
-  int hist_len;        /* energies of the first hist_len steps go to hist (0: none) */
+  int stride;          /* vector-extrapolation stride in steps (0: none) */
+  int hist_len;        /* energies entering the first hist_len steps go to hist (NaN between the
+                          stopping rule's checkpoints, where no energy is formed; 0: none) */
   double *hist;
   double *out;         /* device, HGRID_B2D_NOUT doubles: energy, steps taken, step at which tol
-                          was met (max_steps if never), start and end of the solve on the
-                          device's constant 100 MHz clock */
+                          was met (max_steps if never, -2 if the energy became non-finite), start
+                          and end of the solve on the device's constant 100 MHz clock,
+                          extrapolations done */
 } hgrid_b2d_job;
-#define HGRID_B2D_NOUT 5
+#define HGRID_B2D_NOUT 6
 
 int hgrid_b2d_supported(long n);   /* n in {64, 96, 128, 192, 256} */
 int hgrid_b2d_cus(void);            /* compute units of the device */
 /* jobs: device array; queue: device int (reset by the call).  Tile choices: variant 0/1
- * (threads per pencil for n = 64/128/256), nthreads 256/512 per workgroup (pencils per
- * sweep chunk = nthreads / threads per pencil), nblocks workgroups (persistent: each takes
- * solves from the queue until it is empty). */
-int hgrid_b2d_launch(const hgrid_b2d_job *jobs, int njobs, int *queue, int variant, int nthreads, int nblocks);
+ * (threads per pencil for n = 64 and 256), nthreads 512 or 1024 per workgroup, per_cu
+ * workgroups per CU (2 only with 512 threads; sets the register budget), nblocks workgroups
+ * (persistent: each takes solves from the queue until it is empty). */
+int hgrid_b2d_launch(const hgrid_b2d_job *jobs, int njobs, int *queue, int variant, int nthreads, int per_cu,
+                     int nblocks);
 
 /* ---- warm-start network kernels (BF16 activations, FP32 accumulation) ---------------------
  * No counterpart in the reference.  Activations are channels-last [voxel][channel] BF16 with a

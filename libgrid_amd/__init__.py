@@ -292,6 +292,8 @@ PROTOTYPES = {
     "grid_batch2d_autotune": (C.c_char_p, [L, C.POINTER(WF), C.POINTER(RG), D, L]),
     "grid_batch2d_solve_times": (L, [C.POINTER(D), C.POINTER(D)]),
     "grid_batch2d_set_straggler_cap": (V, [L]),
+    "grid_batch2d_set_extrapolation": (V, [L]),
+    "grid_batch2d_last_extrapolations": (L, []),
     "grid_hip_graphs": (V, [C.c_int]),
     "grid_hip_pass_timing": (V, [C.c_int]),
     "grid_hip_pass_times": (L, [L, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint), C.POINTER(L),

@@ -10,12 +10,13 @@ extern "C" int hgrid_b2d_cus(void) {
   return n;
 }
 
-extern "C" int hgrid_b2d_launch(const hgrid_b2d_job *jobs, int njobs, int *queue, int variant, int nthreads, int nblocks) {
+extern "C" int hgrid_b2d_launch(const hgrid_b2d_job *jobs, int njobs, int *queue, int variant, int nthreads, int per_cu,
+                                int nblocks) {
   if (njobs <= 0) return 0;
   if (nblocks <= 0) return -1;
   if (lg::check(hipGetLastError(), "before batch2d")) return -1;
   if (hgrid_memset(queue, 0, sizeof(int))) return -1;
-  if (variant == 0) return lg::b2d::launch_v0(jobs, njobs, queue, nthreads, nblocks);
-  if (variant == 1) return lg::b2d::launch_v1(jobs, njobs, queue, nthreads, nblocks);
+  if (variant == 0) return lg::b2d::launch_v0(jobs, njobs, queue, nthreads, per_cu, nblocks);
+  if (variant == 1) return lg::b2d::launch_v1(jobs, njobs, queue, nthreads, per_cu, nblocks);
   return -1;
 }

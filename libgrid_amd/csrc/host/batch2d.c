@@ -5,7 +5,8 @@
  *
  * Per solve this is the reference composition (src/grid_wf.c:379-432 in imaginary time,
  * then grid_wf_normalize :694-700, with grid_wf_energy_fft src/grid_wf_fft.c:101-129 as
- * the convergence measure) repeated until the energy change per step is below tol.
+ * the convergence measure) repeated until the energy is converged to tol by the rule of the
+ * host relaxation (relax_rule.h), with that relaxation's vector extrapolation.
  */
 #include <sys/stat.h>
 #include <unistd.h>
@@ -16,12 +17,14 @@ typedef struct {
   int variant, nthreads, per_cu; /* tile variant, workgroup size, persistent workgroups per CU */
 } b2d_tile;
 
-static const b2d_tile kCandidates[] = {{0, 256, 1}, {0, 256, 2}, {0, 512, 1}, {1, 256, 1}, {1, 256, 2}, {1, 512, 1}};
+static const b2d_tile kCandidates[] = {{0, 512, 1}, {0, 512, 2}, {0, 1024, 1}, {1, 512, 1}, {1, 512, 2}, {1, 1024, 1}};
 #define NCAND ((int)(sizeof(kCandidates) / sizeof(kCandidates[0])))
 
 static b2d_tile g_tile = {-1, 0, 0};
-static INT g_cap = 2000;            /* persistent-launch step cap (0: none) */
+static INT g_cap = 0;               /* persistent-launch step cap (0: none, the default) */
+static INT g_stride = 64;           /* vector-extrapolation stride inside the persistent launch (0: plain) */
 static INT g_phase2_stride = 64;    /* vector-extrapolation stride of the straggler phase (0: plain) */
+static INT g_last_extrap = 0;       /* extrapolations done inside the last launch, all solves */
 static INT g_phase2_count = 0;      /* solves continued on the multi-kernel path by the last call */
 static double g_phase2_seconds = 0.0;
 static char g_report[2048];
@@ -82,7 +85,8 @@ typedef struct {
   double *out;
   float *psi_pool, *pv_pool;
   double *v_pool;
-  void *tables; /* kt, k2, tw per solve, packed */
+  float *snap_pool; /* 2 x float2 per point: extrapolation snapshot and difference (NULL: none) */
+  void *tables; /* kh, kf, k2, tw per solve, packed */
   long *off;    /* element offset of each solve in the pools */
   float *host_psi0;
   long total;
@@ -102,6 +106,7 @@ static void b2d_free(b2d_batch *b) {
   hgrid_free(b->psi_pool);
   hgrid_free(b->pv_pool);
   hgrid_free(b->v_pool);
+  if (b->snap_pool) hgrid_free(b->snap_pool);
   hgrid_free(b->tables);
   if (b->hist) hgrid_free(b->hist);
   free(b->hjobs);
@@ -114,7 +119,7 @@ static int cmp_desc(const void *a, const void *b) {
   return (x[0] < y[0]) - (x[0] > y[0]);
 }
 
-static void b2d_pack(b2d_batch *b, INT ns, wf **gwf, rgrid **pot, REAL dt, REAL tol, INT max_steps) {
+static void b2d_pack(b2d_batch *b, INT ns, wf **gwf, rgrid **pot, REAL dt, REAL tol, INT max_steps, INT stride) {
   memset(b, 0, sizeof(*b));
   lg_ensure_device();
   b->nsolves = ns;
@@ -128,7 +133,7 @@ static void b2d_pack(b2d_batch *b, INT ns, wf **gwf, rgrid **pot, REAL dt, REAL 
     if (gwf[i]->propagator != WF_2ND_ORDER_FFT) lg_fatal("batched 2-D solves need WF_2ND_ORDER_FFT");
     b->off[i] = total;
     total += g->ny * g->nz;
-    tab_bytes += g->ny * (4 + 8 + 16);
+    tab_bytes += g->ny * (4 + 4 + 8 + 8);
   }
   b->off[ns] = total;
   b->total = total;
@@ -140,6 +145,8 @@ static void b2d_pack(b2d_batch *b, INT ns, wf **gwf, rgrid **pot, REAL dt, REAL 
   lg_hip(hgrid_malloc((void **)&b->pv_pool, sizeof(float) * total), "hipMalloc (batch pv)");
   lg_hip(hgrid_malloc((void **)&b->v_pool, sizeof(double) * total), "hipMalloc (batch v)");
   lg_hip(hgrid_malloc(&b->tables, tab_bytes), "hipMalloc (batch tables)");
+  if (stride > 0)
+    lg_hip(hgrid_malloc((void **)&b->snap_pool, sizeof(float) * 4 * total), "hipMalloc (batch extrapolation snapshots)");
   lg_hip(hgrid_malloc((void **)&b->out, sizeof(double) * HGRID_B2D_NOUT * ns), "hipMalloc (batch out)");
   lg_hip(hgrid_malloc((void **)&b->queue, 256), "hipMalloc (batch queue)");
   b->hist_len = g_hist_len;
@@ -169,35 +176,45 @@ static void b2d_pack(b2d_batch *b, INT ns, wf **gwf, rgrid **pot, REAL dt, REAL 
       hv[o + k] = vv;
       hpv[o + k] = (float)exp(-dt * vv / HBAR);
     }
-    /* tables: kt (float), k2 (double), tw (double2) */
-    float *kt = (float *)(htab + toff);
-    double *k2 = (double *)(htab + toff + 4 * n);
-    double *tw = (double *)(htab + toff + 12 * n);
+    /* tables: kh, kf (float), k2 (double), tw (float2); k_m with Nyquist at +n/2
+     * (src/grid_wf_fft.c:173-187), factors and twiddles each rounded once */
+    float *kh = (float *)(htab + toff);
+    float *kf = (float *)(htab + toff + 4 * n);
+    double *k2 = (double *)(htab + toff + 8 * n);
+    float *tw = (float *)(htab + toff + 16 * n);
     const double l = 2.0 * M_PI / (g->step * (double)n);
     for (long m = 0; m < n; ++m) {
       const double k = (m <= n / 2) ? (double)m * l : (double)(m - n) * l;
       k2[m] = k * k;
-      kt[m] = (float)(exp(-0.5 * dt * HBAR * k * k / (2.0 * gwf[i]->mass)) / (double)n);
+      kh[m] = (float)exp(-0.5 * dt * HBAR * k * k / (2.0 * gwf[i]->mass));
+      kf[m] = (float)exp(-dt * HBAR * k * k / (2.0 * gwf[i]->mass));
       const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
-      tw[2 * m] = (double)cosl(a);
-      tw[2 * m + 1] = (double)(-sinl(a));
+      tw[2 * m] = (float)cosl(a);
+      tw[2 * m + 1] = (float)(-sinl(a));
     }
     hgrid_b2d_job *j = &b->hjobs[r];
     j->psi = b->psi_pool + 2 * o;
     j->pv = b->pv_pool + o;
     j->v = b->v_pool + o;
-    j->kt = (const float *)((char *)b->tables + toff);
-    j->k2 = (const double *)((char *)b->tables + toff + 4 * n);
-    j->tw = (const void *)((char *)b->tables + toff + 12 * n);
+    j->kh = (const float *)((char *)b->tables + toff);
+    j->kf = (const float *)((char *)b->tables + toff + 4 * n);
+    j->k2 = (const double *)((char *)b->tables + toff + 8 * n);
+    j->tw = (const void *)((char *)b->tables + toff + 16 * n);
+    j->snap = b->snap_pool ? (void *)(b->snap_pool + 4 * o) : NULL;
+    j->diff = b->snap_pool ? (void *)(b->snap_pool + 4 * o + 2 * nn) : NULL;
+    j->stride = b->snap_pool ? (int)stride : 0;
     j->dA = g->step * g->step;
-    j->tfac = HBAR * HBAR / (2.0 * gwf[i]->mass) * g->step * g->step / ((double)n * (double)n);
+    /* E = (tfac sum k^2 |psi_k|^2 + sum V |psi|^2) / sum |psi|^2 for the kernel's unnormalised
+     * transforms (the k-space sum of n^2 points carries n^2 |psi|^2 by Parseval, the real-space
+     * sums n^4 |psi|^2 after the two round trips) */
+    j->tfac = HBAR * HBAR / (2.0 * gwf[i]->mass) * (double)nn;
     j->tol = tol;
     j->n = (int)n;
     j->max_steps = (int)max_steps;
     j->out = b->out + HGRID_B2D_NOUT * i;
     j->hist_len = (int)b->hist_len;
     j->hist = b->hist ? b->hist + b->hist_len * i : NULL;
-    toff += n * (4 + 8 + 16);
+    toff += n * (4 + 4 + 8 + 8);
   }
   free(order);
   lg_hip(hgrid_memcpy_h2d(b->psi_pool, hpsi, sizeof(float) * 2 * total), "hipMemcpy (batch psi)");
@@ -225,7 +242,8 @@ static double b2d_run(b2d_batch *b, const b2d_tile *t) {
   lg_hip(hgrid_event_create(&e0), "event");
   lg_hip(hgrid_event_create(&e1), "event");
   lg_hip(hgrid_event_record(e0), "event");
-  int rc = hgrid_b2d_launch(b->djobs, (int)b->nsolves, b->queue, t->variant, t->nthreads, b2d_blocks(t, b->nsolves));
+  int rc = hgrid_b2d_launch(b->djobs, (int)b->nsolves, b->queue, t->variant, t->nthreads, t->per_cu,
+                            b2d_blocks(t, b->nsolves));
   if (rc) lg_fatal("batched 2-D launch failed: %s", hgrid_last_error());
   lg_hip(hgrid_event_record(e1), "event");
   float ms = 0.f;
@@ -277,7 +295,7 @@ INT grid_batch2d_solve_times(REAL *start, REAL *end) {
 
 const char *grid_batch2d_autotune(INT nsolves, wf **gwf, rgrid **potential, REAL dt, INT probe_steps) {
   b2d_batch b;
-  b2d_pack(&b, nsolves, gwf, potential, dt, 0.0 /* never converge */, probe_steps);
+  b2d_pack(&b, nsolves, gwf, potential, dt, 0.0 /* never converge */, probe_steps, g_stride);
   double best = 1e300;
   int bi = 0;
   size_t pos = (size_t)snprintf(g_report, sizeof(g_report), "batch2d autotune on %s (%ld solves, %ld steps):",
@@ -326,18 +344,20 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
     /* first use on this device: tune on (up to) the first 64 problems for 20 steps */
     grid_batch2d_autotune(nsolves < 64 ? nsolves : 64, gwf, potential, dt, 20);
   }
-  /* Phase 1: the persistent launch, every solve capped at g_cap steps.  A solve that has not
-   * converged by then is a nearly degenerate problem with a long imaginary-time tail; in one
-   * workgroup it would hold the whole launch (the makespan of a 256-problem batch was a single
-   * 22 000-step solve at 3.5% utilisation).  Phase 2 continues those few on the multi-kernel
-   * FP32 path, which puts the whole GPU on one grid. */
+  /* The persistent launch relaxes every solve to convergence; nearly degenerate problems, whose
+   * imaginary-time tails are long, are shortened by the launch's own vector extrapolation
+   * (grid_batch2d_set_extrapolation).  With a step cap (grid_batch2d_set_straggler_cap, off by
+   * default) solves unconverged at the cap continue one at a time on the multi-kernel FP32 path,
+   * which puts the whole GPU on one grid. */
   const INT cap = (g_cap > 0 && g_cap < max_steps) ? g_cap : max_steps;
   b2d_batch b;
-  b2d_pack(&b, nsolves, gwf, potential, dt, tol, cap);
+  b2d_pack(&b, nsolves, gwf, potential, dt, tol, cap, g_stride);
   const double ms = b2d_run(&b, &g_tile);
   double *out = (double *)malloc(sizeof(double) * HGRID_B2D_NOUT * nsolves);
   lg_hip(hgrid_memcpy_d2h(out, b.out, sizeof(double) * HGRID_B2D_NOUT * nsolves), "hipMemcpy (batch out)");
   solve_times(out, nsolves);
+  g_last_extrap = 0;
+  for (INT i = 0; i < nsolves; ++i) g_last_extrap += (INT)out[HGRID_B2D_NOUT * i + 5];
   if (b.hist) {
     free(g_hist);
     g_hist = (double *)malloc(sizeof(double) * (size_t)(b.hist_len * nsolves));
@@ -385,6 +405,13 @@ REAL grid_wf_batch2d_relax(INT nsolves, wf **gwf, rgrid **potential, REAL dt, RE
 }
 
 void grid_batch2d_set_straggler_cap(INT steps) { g_cap = steps; }
+
+void grid_batch2d_set_extrapolation(INT stride) {
+  if (stride < 0) lg_fatal("grid_batch2d_set_extrapolation: stride %ld is negative", (long)stride);
+  g_stride = stride;
+}
+
+INT grid_batch2d_last_extrapolations(void) { return g_last_extrap; }
 
 void grid_batch2d_set_straggler_extrapolation(INT stride) {
   if (stride < 0 || stride % 16) lg_fatal("grid_batch2d_set_straggler_extrapolation: stride %ld is not a multiple of 16", (long)stride);

@@ -1,7 +1,7 @@
 """Batched 2-D imaginary-time ground states in one persistent launch (complex64 storage, FP64
-reductions) against the FP64 path on the same problems: the FP64 reference relaxes each
-problem with grid_wf_gpe_relax (g = 0), which is pinned to the oracle in test_gpu_parity.py,
-and a few problems are relaxed by the CPU oracle directly."""
+reductions, vector extrapolation inside the launch) against the FP64 path on the same problems:
+the FP64 reference relaxes each problem with grid_wf_gpe_relax (g = 0), which is pinned to the
+oracle in test_gpu_parity.py, and a few problems are relaxed by the CPU oracle directly."""
 import ctypes as C
 import sys
 import time
@@ -63,11 +63,19 @@ def fp64_reference(lg, problem, tol=1e-10, max_steps=MAX_STEPS):
     return e.value, steps, out
 
 
-def test_batch2d_small_vs_oracle_and_fp64(lg, orc):
+@pytest.mark.parametrize("stride", [64, 0])
+def test_batch2d_small_vs_oracle_and_fp64(lg, orc, stride):
     problems = W.wells_2d(seed=7, count=6, sizes=(64, 96))
-    case(f"batch2d small: sizes {[p[2]['n'] for p in problems]}")
+    case(f"batch2d small (extrapolation stride {stride}): sizes {[p[2]['n'] for p in problems]}")
     wfs, pots = make(lg, problems)
-    secs, e, st = run_batch(lg, wfs, pots, 1e-8, MAX_STEPS)
+    lg.grid_batch2d_set_extrapolation(stride)
+    try:
+        secs, e, st = run_batch(lg, wfs, pots, 1e-8, MAX_STEPS)
+    finally:
+        lg.grid_batch2d_set_extrapolation(64)
+    print(f"extrapolations: {lg.grid_batch2d_last_extrapolations()}")
+    if stride == 0:
+        assert lg.grid_batch2d_last_extrapolations() == 0
     for i, (psi, v, prm) in enumerate(problems):
         e64, s64, st64 = fp64_reference(lg, problems[i])
         got = LG.get_cgrid(wfs[i].contents.grid)
@@ -112,7 +120,10 @@ def test_batch2d_256_solves(lg):
           f"converged {conv}/{ns}; steps min/median/max {st.min()}/{int(np.median(st))}/{st.max()}")
     p2n, p2s = C.c_long(), C.c_double()
     lg.grid_batch2d_last_phase2(C.byref(p2n), C.byref(p2s))
-    print(f"straggler phase: {p2n.value} solves continued on the multi-kernel FP32 path in {p2s.value:.3f} s")
+    print(f"straggler phase: {p2n.value} solves continued on the multi-kernel FP32 path in {p2s.value:.3f} s; "
+          f"extrapolations inside the launch: {lg.grid_batch2d_last_extrapolations()}")
+    # every solve finishes inside the one persistent launch
+    assert p2n.value == 0
     t_s = (C.c_double * ns)()
     t_e = (C.c_double * ns)()
     assert lg.grid_batch2d_solve_times(t_s, t_e) == ns
