@@ -238,7 +238,8 @@ def fp32_512_bench(lib, LG, steps=20, warmup=3):
     """The 512^3 complex64 real-time GPE step of the vortex-ring workload (dt 0.02, g 0.1,
     harmonic trap omega 0.1, spacing 0.15): FP64 twiddles, arguments and reductions.  The step's
     cost does not depend on the state, so a Gaussian condensate built on the host stands in for
-    the ring (its accuracy run is tests/test_gpu_configs.py)."""
+    the ring (its accuracy run is tests/test_gpu_configs.py).  Steps/s, the step's fraction of the
+    HBM roofline on the bytes it moves, and every pass kind timed in place with HIP events."""
     n, step, g, dt, omega = 512, 0.15, 0.1, 0.02, 0.1
     w = lib.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"ring")
     vr = lib.rgrid_alloc(n, n, n, step, LG.boundary("rgrid_value_outside_periodic"), None, b"trap")
@@ -248,23 +249,30 @@ def fp32_512_bench(lib, LG, steps=20, warmup=3):
     del r2
     lib.cgrid_set_precision(w.contents.grid, b"\x01")
     lib.grid_wf_gaussian_start(w, 10.0)
-    for _ in range(warmup):
+
+    def one():
         lib.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(dt), b"\x00")
+
+    for _ in range(warmup):
+        one()
     lib.grid_hip_sync()
     t = LG.HipTimer()
     t.start()
     for _ in range(steps):
-        lib.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(dt), b"\x00")
+        one()
     ms = t.stop() / steps
     t.close()
+    N = n ** 3
+    passes = insitu_passes(lib, LG, one, 3, N, 8)
+    actual = sum(v["bytes"] * v["count"] for v in passes.values() if "bytes" in v)
     lib.grid_wf_free(w)
     lib.rgrid_free(vr)
-    N = n ** 3
-    model = 200 * N  # (4 FFTs x 3 sweeps x 2 x 8 B + 8 B potential) per point
     out = dict(workload="vortex_ring 512^3 complex64 real-time GPE step (dt 0.02, g 0.1)", ms_per_step=ms,
-               steps_per_s=1000.0 / ms, model_bytes_per_step=model,
-               model_frac=model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS)
-    log(f"fp32 512^3: {ms:.3f} ms/step, {1000 / ms:.1f} steps/s, model-roofline frac {out['model_frac']:.3f}")
+               steps_per_s=1000.0 / ms, actual_bytes_per_step=actual, actual_GBps=actual / (ms * 1e-3) / 1e9,
+               frac=actual / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, passes=passes)
+    log(f"fp32 512^3: {ms:.3f} ms/step, {1000 / ms:.1f} steps/s, {out['actual_GBps']:.0f} GB/s on {actual / N:.0f} B/pt "
+        f"({out['frac']:.3f} of 8 TB/s); passes " +
+        ", ".join(f"{k} {v['ms'] * 1e3:.0f} us {v.get('frac', 0):.2f}" for k, v in passes.items()))
     return out
 
 
@@ -343,13 +351,14 @@ def batch2d_bench(lib, LG, count=256, tol=1e-8, dt=0.1):
         lib.grid_wf_free(w)
     for r in pots:
         lib.rgrid_free(r)
-    out = dict(workload=f"{count} x 2-D wells (n in 64..256, spacing 0.2, dt {dt}, tol {tol})", solves_per_s=count / secs,
-               launch_s=secs, wall_s_incl_packing=wall, steps_median=int(np.median(steps)), steps_max=int(steps.max()),
-               persistent_launch_s=secs - p2s.value, straggler_solves=int(p2n.value), straggler_s=p2s.value,
-               utilisation_persistent=busy / (cus * (secs - p2s.value)) if cus > 0 else None, autotune=report)
-    log(f"batch2d: {count / secs:.1f} solves/s ({secs - p2s.value:.3f} s persistent launch + {p2n.value} stragglers "
-        f"{p2s.value:.3f} s), steps median {out['steps_median']} max {out['steps_max']}, launch utilisation "
-        f"{out['utilisation_persistent']:.3f}")
+    out = dict(workload=f"{count} x 2-D wells (n in 64..256, spacing 0.2, dt {dt}, tol {tol}), one persistent launch",
+               solves_per_s=count / secs, launch_s=secs, wall_s_incl_packing=wall, steps_median=int(np.median(steps)),
+               steps_max=int(steps.max()), extrapolations=int(lib.grid_batch2d_last_extrapolations()),
+               straggler_solves=int(p2n.value), straggler_s=p2s.value, busy_workgroup_s=busy,
+               utilisation=busy / (cus * secs) if cus > 0 else None, autotune=report)
+    log(f"batch2d: {count / secs:.1f} solves/s ({secs:.3f} s, one launch, {out['extrapolations']} extrapolations, "
+        f"{p2n.value} stragglers), steps median {out['steps_median']} max {out['steps_max']}, busy {busy:.2f} WG-s, "
+        f"utilisation {out['utilisation']:.3f}")
     return out
 
 

@@ -56,6 +56,24 @@ def test_network_fp32_start_256(lg, net):
         assert s[m]["speedup_total"] > 1.5, m
 
 
+@pytest.mark.timeout(900)
+def test_fast_path_speedup_on_heldout_subset(lg, net):
+    """The time-to-ground-state target on the fixed held-out subset (every 8th of the 200 held-out
+    256^3 potentials, 25 of them): the fast path (network start, FP32 relaxation to 3e-8 Eh, FP64
+    polish to 1e-10 Eh, vector extrapolation in both stages) against plain FP64 relaxation from the
+    Gaussian start, both timed including building the start, both judged by the converged
+    energy."""
+    case(f"warm start fast path vs plain FP64, 256^3 held-out subset {WS.HELDOUT_SUBSET}")
+    modes = ("gauss64", "net32ee")
+    recs = WS.evaluate(lg, net, 0, WS.FINE, modes, indices=WS.HELDOUT_SUBSET)
+    s = WS.summarize(recs, modes)
+    print(s)
+    assert s["gauss64"]["converged"] == s["count"] and s["net32ee"]["converged"] == s["count"]
+    # both stop on the rule's estimate at tol = 1e-10 (spread over all 200: 2.2e-10)
+    assert s["energy_spread_max"] < 3e-10
+    assert s["net32ee"]["speedup_total"] >= 3.0
+
+
 def test_no_network_falls_back_to_gaussian(lg):
     case("warm start fallback")
     n = 32
