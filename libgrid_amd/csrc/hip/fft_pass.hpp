@@ -161,6 +161,7 @@ template <typename C, int L, bool CONTIG>
 __device__ __forceinline__ void fft_core2(C (&v)[Split<L>::R], int t, int q, bool inverse, const C *__restrict__ tw,
                                           typename real_of<C>::type *lds) {
   constexpr int TPP = Split<L>::TPP, R = Split<L>::R, P = Geom<C, L, CONTIG>::P;
+  static_assert(!CONTIG || 64 % TPP == 0, "a contiguous pencil's threads lie in one wave");
   if (inverse) {
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = cconj(v[j]);
@@ -174,21 +175,22 @@ __device__ __forceinline__ void fft_core2(C (&v)[Split<L>::R], int t, int q, boo
 #pragma unroll
     for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], twl[t * k1]);
     constexpr int A = R / TPP;
-    // real parts
-    __syncthreads();  // the previous exchange may still be reading
+    // real parts.  A contiguous pencil's TPP threads are consecutive lanes of one wave and its
+    // exchange region is its own, so ordering that wave's LDS accesses is enough there.
+    pencil_sync<CONTIG>();  // the previous exchange may still be reading
 #pragma unroll
     for (int k1 = 0; k1 < R; ++k1) lds[lds_index<CONTIG, R, TPP, P>(q, k1, t)] = v[k1].x;
-    __syncthreads();
+    pencil_sync<CONTIG>();
     typename real_of<C>::type ur[A][TPP];
 #pragma unroll
     for (int a = 0; a < A; ++a)
 #pragma unroll
       for (int s = 0; s < TPP; ++s) ur[a][s] = lds[lds_index<CONTIG, R, TPP, P>(q, t + TPP * a, s)];
-    __syncthreads();
+    pencil_sync<CONTIG>();
     // imaginary parts
 #pragma unroll
     for (int k1 = 0; k1 < R; ++k1) lds[lds_index<CONTIG, R, TPP, P>(q, k1, t)] = v[k1].y;
-    __syncthreads();
+    pencil_sync<CONTIG>();
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       C u[TPP];
