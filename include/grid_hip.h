@@ -167,6 +167,14 @@ int hgrid_step_finalize(const hgrid_finalize *f);
  * (deterministic; scratch must hold hgrid_reduce_scratch(n) doubles). */
 long hgrid_reduce_scratch(long n);
 int hgrid_sum_norm2(double *out, const void *a, int prec, long n, const double *scale, double *scratch);
+/* hgrid_sum_norm2 without scratch: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).  It allocates
+ * nothing: the partials go to a fixed 128 KiB array per device (2 * 8192 + 2 doubles, enough for
+ * every n, since a reduction uses at most 8192 workgroups), looked up once at hgrid_init.  n must
+ * be in 1 .. 2^34, else it returns -1 with an error and launches nothing.  Every call on a device
+ * shares that array, so calls that overlap on one device (other streams, other threads) race;
+ * calls on the library stream are ordered. */
+int hgrid_sum_norm2_noalloc(double *out, const void *a, int prec, long n, const double *scale);
 int hgrid_sum_vnorm2(double *out, const void *a, int prec, const double *v, long n, const double *scale, double *scratch);
 int hgrid_sum_conj_product(double *out2, const void *a, const void *b, int prec, long n, double *scratch);
 /* sum (k2x[i] + k2y[j] + k2z[k]) |a[i,j,k]|^2 over an nx x ny x nz array */

@@ -340,6 +340,9 @@ PROTOTYPES = {
     "hgrid_fft_pass": (C.c_int, [C.POINTER(hgrid_pass)]),
     "hgrid_stream_copy": (C.c_int, [VP, VP, C.c_size_t]),
     "hgrid_pass_blocks": (L, [C.c_int, L, L, L, C.c_int]),
+    "hgrid_reduce_scratch": (L, [L]),
+    "hgrid_sum_norm2": (C.c_int, [VP, VP, C.c_int, L, VP, VP]),
+    "hgrid_sum_norm2_noalloc": (C.c_int, [VP, VP, C.c_int, L, VP]),
 }
 
 

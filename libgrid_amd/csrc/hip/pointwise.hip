@@ -451,6 +451,36 @@ extern "C" int hgrid_sum_norm2(double *out, const void *a, int prec, long n, con
   return hgrid_reduce_partials(out, scratch, nb, 1);
 }
 
+// ---- allocation-free norm reduction ------------------------------------------------------
+// hgrid_sum_norm2 with its partials in a fixed device array instead of caller or library
+// scratch: grid_for() caps a reduction at 8192 workgroups, so 2 * 8192 + 2 doubles (the size
+// hgrid_reduce_scratch gives for the largest grids, 128 KiB) hold the partials of every n.
+// Each device holds its own copy until the process ends; its address is looked up once per
+// device at library set-up (hgrid_init), and host code uses only that address.
+constexpr long kNoAllocScratch = 2L * 8192 + 2;
+constexpr long kNoAllocMaxN = 1L << 34;  // elements: far beyond any grid that fits in HBM
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_dev[kMaxDevices];
+
+int lg::noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail("allocation-free scratch: device index out of range");
+  if (g_scratch_dev[device]) return 0;
+  void *p = nullptr;
+  if (check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress (allocation-free scratch)")) return -1;
+  g_scratch_dev[device] = (double *)p;
+  return 0;
+}
+
+extern "C" int hgrid_sum_norm2_noalloc(double *out, const void *a, int prec, long n, const double *scale) {
+  if (n < 1 || n > kNoAllocMaxN) return fail("hgrid_sum_norm2_noalloc: n must be in 1 .. 2^34");
+  static_assert(2L * 8192 + 2 <= kNoAllocScratch, "scratch holds the partials of the largest reduction");
+  int dev = 0;
+  if (check(hipGetDevice(&dev), "hipGetDevice")) return -1;
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_dev[dev]) return fail("hgrid_sum_norm2_noalloc: call hgrid_init first");
+  return hgrid_sum_norm2(out, a, prec, n, scale, g_scratch_dev[dev]);
+}
+
 extern "C" int hgrid_sum_vnorm2(double *out, const void *a, int prec, const double *v, long n, const double *scale,
                                 double *scratch) {
   const unsigned nb = grid_for(n);

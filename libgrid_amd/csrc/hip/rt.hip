@@ -56,6 +56,8 @@ extern "C" int hgrid_init(int device) {
   }
   if (check(hipStreamCreateWithFlags(&lg::g_stream, hipStreamNonBlocking), "hipStreamCreate")) return -1;
   lg::g_device = device;
+  // the device's fixed scratch of the allocation-free entry point
+  if (lg::noalloc_setup(device)) return -1;
   return 0;
 }
 

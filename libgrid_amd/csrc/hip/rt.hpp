@@ -9,4 +9,6 @@ int check(hipError_t e, const char *what);
 int check_launch(const char *what);
 // Record an argument error (nothing was launched); returns -1.
 int fail(const char *what);
+// Look up this device's allocation-free reduction scratch (pointwise.hip); 0 or -1.
+int noalloc_setup(int device);
 }  // namespace lg
