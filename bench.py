@@ -408,13 +408,21 @@ def summarize_rocpd(stats_dir, fetch_dir, write_dir, command):
             for name, v, n in c.execute("select kernel_name, avg(value), count(*) from counters_collection where counter_name = ? "
                                         "group by kernel_name", (tag,)):
                 vals.setdefault(name, {})[tag] = (v, n)
-        copy = [k for k in vals if "k_stream_copy" in k]
-        if copy:
-            cf, cw = vals[copy[0]]["FETCH_SIZE"][0], vals[copy[0]]["WRITE_SIZE"][0]
-            grid_kib = 256.0 ** 3 * 16 / 1024.0
-            ff, fw = grid_kib / cf, grid_kib / cw
+        # the copy runs on one 256 MiB grid and on a 2 GiB buffer: calibrate on the 256 MiB
+        # dispatches (the smaller of the two groups of counter values, eight times apart)
+        grid_kib = 256.0 ** 3 * 16 / 1024.0
+        factors = {}
+        for tag, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+            v = sorted(r[0] for r in db(d).execute("select value from counters_collection where counter_name = ? and "
+                                                   "kernel_name like '%k_stream_copy%'", (tag,)))
+            if v:
+                small = [x for x in v if x < 2.83 * v[0]]
+                factors[tag] = grid_kib / (sum(small) / len(small))
+        if len(factors) == 2:
+            ff, fw = factors["FETCH_SIZE"], factors["WRITE_SIZE"]
             out.append(f"calibration: stream copy of one 256^3 complex128 grid ({grid_kib:.0f} KiB each way) reads "
-                       f"FETCH_SIZE {cf:.0f} KiB, WRITE_SIZE {cw:.0f} KiB -> factors fetch x{ff:.3f}, write x{fw:.3f}")
+                       f"FETCH_SIZE {grid_kib / ff:.0f} KiB, WRITE_SIZE {grid_kib / fw:.0f} KiB -> factors fetch x{ff:.3f}, "
+                       f"write x{fw:.3f}")
             out.append("HBM bytes per dispatch (calibrated): MiB read, MiB written, dispatches  kernel")
             for name in sorted(vals):
                 if "FETCH_SIZE" not in vals[name] or "WRITE_SIZE" not in vals[name]:
