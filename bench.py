@@ -84,8 +84,8 @@ def insitu_passes(lib, LG, step_fn, steps, N, S):
 
 
 def copy_ceiling(lib, nbytes, reps=10):
-    """hgrid_stream_copy (16-byte loads and stores, eight per thread) of one buffer into another:
-    read + write bytes over the average time."""
+    """hgrid_stream_copy (one 16-byte load and store per thread, the fastest copy shape measured on
+    MI355X) of one buffer into another: read + write bytes over the average time."""
     a, b = C.c_void_p(), C.c_void_p()
     if lib.hgrid_malloc(C.byref(a), nbytes) or lib.hgrid_malloc(C.byref(b), nbytes):
         raise RuntimeError(lib.hgrid_last_error().decode())
@@ -460,6 +460,10 @@ def main():
                     help="held-out 256^3 wells potentials for the warm-start time-to-ground-state comparison: "
                          "the first N in all five modes, or -1 (default) for the fixed subset of every 8th "
                          "of the 200 (25 potentials) in the plain and fast modes, 0 to skip")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch relaxation batches eagerly instead of replaying HIP graphs (for profiling: "
+                         "rocprofv3 --kernel-trace has crashed inside hipGraphLaunch once a process had destroyed "
+                         "one graph and instantiated another)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -471,6 +475,8 @@ def main():
 
     lib = LG.load()
     lib.grid_hip_enable(b"\x01", 1, None)
+    if args.no_graphs:
+        lib.grid_hip_graphs(0)
     log(f"device {lib.hgrid_device_name().decode()}")
     n = args.n
     t0 = time.perf_counter()

