@@ -299,6 +299,11 @@ INT grid_wf_gpe_relax_extrapolated(wf *gwf, rgrid *vext, REAL g, REAL dt, REAL t
 /* Relaxation replays batches of 16 steps as a HIP graph after the first batch (default on);
  * grid_hip_graphs(0) launches every step eagerly instead. */
 void grid_hip_graphs(int enable);
+/* Linear relaxations (g = 0) of 3-D grids with WF_2ND_ORDER_FFT merge the last kinetic half-step
+ * of each step with the first of the next inside a batch (default on): four passes per step
+ * instead of seven, the same states at the batch boundaries.  grid_hip_merged_relaxation(0)
+ * runs whole steps. */
+void grid_hip_merged_relaxation(int enable);
 /* Per-pass timing (this build's addition; plays the role of the reference's cuda_statistics,
  * src/cuda.c): grid_hip_pass_timing(1) starts recording every FFT/propagation pass kernel
  * between two HIP events on the library stream (up to 4096 passes; enabling again resets the

@@ -295,6 +295,7 @@ PROTOTYPES = {
     "grid_batch2d_set_extrapolation": (V, [L]),
     "grid_batch2d_last_extrapolations": (L, []),
     "grid_hip_graphs": (V, [C.c_int]),
+    "grid_hip_merged_relaxation": (V, [C.c_int]),
     "grid_hip_pass_timing": (V, [C.c_int]),
     "grid_hip_pass_times": (L, [L, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint), C.POINTER(L),
                                 C.POINTER(C.c_int), C.POINTER(D)]),

@@ -43,17 +43,23 @@ enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3 };
 
 // Threads per pencil (TPP) and values per thread (R) for each supported length L = R * TPP:
 // powers of two up to 1024 and 3 * 2^k up to 384.  Two register stages (R-point DFTs, then
-// R / TPP TPP-point DFTs per thread) need R to be a multiple of TPP; L = 512 instead runs three
-// 8-point stages with 64 threads per pencil (THREE), which keeps 8 values per thread instead of
-// 32 (the two-stage 16 x 32 split held 32 values through two transforms and ran at 1-2
-// waves/SIMD).
-template <int L> struct Split {
-  static constexpr bool THREE = L == 512;
+// R / TPP TPP-point DFTs per thread) need R to be a multiple of TPP.  Contiguous 512-point
+// pencils instead run three 8-point stages with 64 threads per pencil (THREE): a pencil is then
+// one wave, which orders its exchanges itself, and keeps 8 values per thread instead of 32 (the
+// two-stage 16 x 32 split ran the contiguous GPE pass at 1 wave/SIMD).  The same form for
+// contiguous 256-point pencils (8 x 8 x 4, 32 threads) was measured too: it cut the FP64 passes'
+// registers (GPE 234 -> 152) but made the 256^3 GPE step slower (0.822 against 0.816 ms: the GPE
+// pass 124 -> 136 us), so 256 keeps the 16 x 16 split.  Strided 512-point pencils keep the
+// two-stage split as well: their three-stage form spans 16 waves per workgroup, and its eight
+// workgroup barriers per two-transform pass cost more than the registers save (x kinetic pass
+// at 512^3 FP32: 1156 us three-stage, 810 us two-stage).
+template <int L, bool CONTIG> struct Split {
+  static constexpr bool THREE = L == 512 && CONTIG;
   static constexpr int TPP = THREE ? 64
                              : (L <= 16 || L == 24) ? 1
                              : L == 32 || L == 48 || L == 96 ? 4
                              : L == 64 || L == 128 || L == 192 || L == 384 ? 8
-                             : L == 256 ? 16
+                             : L == 256 || L == 512 ? 16
                              : 32;
   static constexpr int R = L / TPP;
   static_assert(R * TPP == L, "unsupported length");
@@ -65,14 +71,13 @@ template <int L> struct Split {
   X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
 
 template <typename C, int L, bool CONTIG> struct Geom {
-  static constexpr int TPP = Split<L>::TPP;
-  static constexpr int R = Split<L>::R;
-  static constexpr bool THREE = Split<L>::THREE;
+  static constexpr int TPP = Split<L, CONTIG>::TPP;
+  static constexpr int R = Split<L, CONTIG>::R;
+  static constexpr bool THREE = Split<L, CONTIG>::THREE;
   // workgroup size: two-stage lengths take 256 threads unless one real half of the LDS exchange
-  // exceeds 32 KiB.  Three-stage: contiguous pencils are one wave each, four per workgroup;
-  // strided workgroups take the pencils of 128 consecutive bytes of a row (16 FP32 / 8 FP64).
-  static constexpr int NT = THREE ? (CONTIG ? 256 : TPP * (int)(128 / sizeof(C)))
-                                  : (TPP > 1 && (long)R * (long)sizeof(C) * 128 > 32768) ? 128 : 256;
+  // exceeds 32 KiB.  Three-stage (contiguous only): pencils are one wave each, four per
+  // workgroup.
+  static constexpr int NT = THREE ? 256 : (TPP > 1 && (long)R * (long)sizeof(C) * 128 > 32768) ? 128 : 256;
   static constexpr int P = NT / TPP;  // pencils per workgroup
   // complex entries per pencil in the three-stage exchange buffer (512 + 4: consecutive pencils
   // start 4 entries apart in the LDS banks)
@@ -158,9 +163,9 @@ __device__ __forceinline__ int lds_index(int q, int k1, int t) {
 
 // One L-point transform in two register stages (the decomposition at the top of this file).
 template <typename C, int L, bool CONTIG>
-__device__ __forceinline__ void fft_core2(C (&v)[Split<L>::R], int t, int q, bool inverse, const C *__restrict__ tw,
+__device__ __forceinline__ void fft_core2(C (&v)[(Split<L, CONTIG>::R)], int t, int q, bool inverse, const C *__restrict__ tw,
                                           typename real_of<C>::type *lds) {
-  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, P = Geom<C, L, CONTIG>::P;
+  constexpr int TPP = Split<L, CONTIG>::TPP, R = Split<L, CONTIG>::R, P = Geom<C, L, CONTIG>::P;
   static_assert(!CONTIG || 64 % TPP == 0, "a contiguous pencil's threads lie in one wave");
   if (inverse) {
 #pragma unroll
@@ -210,9 +215,9 @@ __device__ __forceinline__ void fft_core2(C (&v)[Split<L>::R], int t, int q, boo
 // ---- one L-point transform of the pencil owned by (q, t) ----------------------------------
 
 template <typename C, int L, bool CONTIG>
-__device__ __forceinline__ void fft_core(C (&v)[Split<L>::R], int t, int q, bool inverse, const C *__restrict__ tw,
+__device__ __forceinline__ void fft_core(C (&v)[(Split<L, CONTIG>::R)], int t, int q, bool inverse, const C *__restrict__ tw,
                                          typename real_of<C>::type *lds) {
-  if constexpr (Split<L>::THREE) {
+  if constexpr (Split<L, CONTIG>::THREE) {
     fft_core3<C, CONTIG>(v, t, q, inverse, tw, lds);
   } else {
     fft_core2<C, L, CONTIG>(v, t, q, inverse, tw, lds);
@@ -254,7 +259,7 @@ template <typename C, int L, bool CONTIG, int KIND, typename S = C>
 __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(const hgrid_pass a) {
   using Rl = typename real_of<C>::type;
   using G = Geom<C, L, CONTIG>;
-  constexpr int TPP = Split<L>::TPP, R = Split<L>::R, NT = G::NT, P = G::P;
+  constexpr int TPP = Split<L, CONTIG>::TPP, R = Split<L, CONTIG>::R, NT = G::NT, P = G::P;
   constexpr int LDS_N = CONTIG ? G::LDS_CONTIG : G::LDS_STRIDED;
   constexpr unsigned CS = sizeof(S);  // bytes per stored element
   __shared__ __align__(16) Rl lds[LDS_N];
@@ -385,8 +390,15 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
         for (int j = 0; j < R; ++j) v[j] = cscale_d(v[j], ka[t + TPP * j].x * f.x);
       }
     } else {
+      // complex factors (real time): formed in FP64 in groups of four behind scheduling fences,
+      // so that few FP64 table entries and products are live at once
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
+      for (int j0 = 0; j0 < R; j0 += 4) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = j0; j < (j0 + 4 < R ? j0 + 4 : R); ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
     if (fl & HGRID_PF_KCUT) {
       // CFFT: components with |k - k0|^2 >= kmax^2 become exact zeros
@@ -453,6 +465,11 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = cscale(v[j], post);
     }
+  }
+  if constexpr (KIND == KIND_FFT || KIND == KIND_GPE) {
+    // the stored values (a merged relaxation step renormalises from the output of its potential
+    // pass; kinetic and complex-potential passes never reduce their output, and the accumulation
+    // would cost the strided 512-point kinetic pass its second wave per SIMD)
     if (fl & HGRID_PF_POST_RED) {
 #pragma unroll
       for (int j = 0; j < R; ++j) acc[HGRID_RED_POST] += cnorm2(v[j]);

@@ -89,6 +89,7 @@ void cgrid_free(cgrid *grid) {
     if (d->work) lg_hip(hgrid_free(d->work), "hipFree");
     if (d->rho) lg_hip(hgrid_free(d->rho), "hipFree");
     free_ktab(&d->kin);
+    free_ktab(&d->kin2);
     free_ktab(&d->kin_energy);
     free(d);
   }
@@ -165,6 +166,7 @@ static void download(cgrid *g) {
  * (unnormalised inverse: the step's kinetic factor already carried 1/N). */
 void lg_c_materialize(cgrid *g) {
   grid_dev *d = g->dev;
+  if (d->kpend) lg_fatal("internal: a merged relaxation batch left its state between kinetic half-steps");
   if (!d->zpend) return;
   lg_shape s;
   lg_squeeze(g->nx, g->ny, g->nz, &s);
@@ -258,6 +260,7 @@ void cgrid_set_precision(cgrid *grid, char prec) {
     /* FP32 storage either way: only the arithmetic of the passes changes */
     d->prec = p;
     free_ktab(&d->kin);
+    free_ktab(&d->kin2);
     return;
   }
   if (d->d) {
@@ -278,6 +281,7 @@ void cgrid_set_precision(cgrid *grid, char prec) {
       d->work = NULL;
     }
     free_ktab(&d->kin);
+    free_ktab(&d->kin2);
   }
   d->prec = p;
 }

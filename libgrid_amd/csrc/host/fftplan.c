@@ -180,8 +180,12 @@ static void upload_table(void **dptr, const long double *re, const long double *
 }
 
 ktab_cache *lg_kinetic_tables(cgrid *g, double tm_re, double tm_im, double fold) {
+  return lg_kinetic_tables_in(g, &g->dev->kin, tm_re, tm_im, fold);
+}
+
+/* the tables in a given cache slot of the grid (kin, or kin2 for a merged full step) */
+ktab_cache *lg_kinetic_tables_in(cgrid *g, ktab_cache *k, double tm_re, double tm_im, double fold) {
   grid_dev *d = g->dev;
-  ktab_cache *k = &d->kin;
   if (k->valid && k->prec == d->prec && k->tm_re == tm_re && k->tm_im == tm_im && k->kx0 == g->kx0 && k->ky0 == g->ky0 &&
       k->kz0 == g->kz0 && k->step == g->step && k->fold == fold && k->nx == g->nx && k->ny == g->ny && k->nz == g->nz)
     return k;

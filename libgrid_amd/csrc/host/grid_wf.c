@@ -297,6 +297,13 @@ typedef struct {
   int want_energy;       /* reductions for E of the input state */
   int want_post_norm;    /* sum |psi|^2 of the output */
   double renorm_target;  /* > 0: make the output's pending scale renormalise it to this norm */
+  /* the split stages of a merged relaxation batch (3-D deferred steps, linear potential):
+   * 0 a whole step K(t/2) V(t) K(t/2);
+   * 1 its start K(t/2) V(t), from psi;
+   * 2 K(t) V(t): the last half-kinetic factor of one step and the first of the next, merged;
+   * 3 the final K(t/2), back to psi.  Stages 1 and 2 leave the state between kinetic half-steps
+   *   (kpend), which stage 3 resolves before anything else may read the grid. */
+  int stage;
 } step_args;
 
 static void fused_step(wf *gwf, const step_args *sa) {
@@ -313,9 +320,16 @@ static void fused_step(wf *gwf, const step_args *sa) {
    * nothing needs them).  With FP32 butterflies the two roundings fewer per step also lower
    * the error against FP64 (512^3 vortex ring after 500 steps: 1.3e-4 instead of 1.6e-4). */
   const int defer = s.m == 3 && !sa->no_defer;
+  const int stage = sa->stage;
+  if (stage && (!defer || sa->gpe != 1 || sa->g != 0.0 || (stage >= 2 && (!d->zpend || !d->kpend || sa->want_energy))))
+    lg_fatal("internal: merged relaxation stage %d on a state it does not apply to", stage);
+  if (stage <= 1 && d->kpend) lg_fatal("internal: a merged relaxation batch was left unfinished");
   if (d->zpend && !defer) lg_c_materialize(g);
   const int zin = d->zpend;
-  ktab_cache *kt = lg_kinetic_tables(g, sa->tm_re, sa->tm_im, g->fft_norm);
+  /* stage 2 applies the kinetic factor of a whole step: the tables of twice the half-step time
+   * (one unnormalised round trip per step, so one 1/N) */
+  ktab_cache *kt = stage == 2 ? lg_kinetic_tables_in(g, &d->kin2, 2.0 * sa->tm_re, 2.0 * sa->tm_im, g->fft_norm)
+                              : lg_kinetic_tables(g, sa->tm_re, sa->tm_im, g->fft_norm);
   ktab_cache *k2 = (sa->want_energy) ? lg_k2_tables(g) : NULL;
   /* V_eff = vext + g |psi|^2 is materialised only for g != 0; with g = 0 the potential pass
    * reads vext itself */
@@ -353,7 +367,11 @@ static void fused_step(wf *gwf, const step_args *sa) {
     p.cons = sa->cons;
     p.mid = 1.0;
   }
-  if (zin) {
+  if (stage >= 2) {
+    /* no start-of-step pass: the stored transform S was produced by a forward transform along
+     * the contiguous axis (the previous stage's potential pass), so the first y pass reads it
+     * as it is and applies the pending scale */
+  } else if (zin) {
     p.dir1 = +1;
     p.flags |= HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE;
     if (veff || sa->want_energy) lg_run_pass(&p); /* otherwise nothing reads the input state */
@@ -364,7 +382,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
    * of the real-space state IDFT(S) it replaces would have given n2 * S (an unnormalised round
    * trip), which the kinetic factors' 1/N^2 accounts for: that pass applies n2 and the scale */
   const double *yscale = zin ? scale : NULL;
-  int ypost = zin;
+  int ypost = zin && stage < 2;
   /* all reductions of the step and the pending-scale update go into one finalize launch */
   hgrid_finalize fin;
   memset(&fin, 0, sizeof(fin));
@@ -374,6 +392,8 @@ static void fused_step(wf *gwf, const step_args *sa) {
   }
 
   for (int half = 0; half < 2; ++half) {
+    if ((stage == 1 || stage == 2) && half == 1) break; /* the step ends after its potential */
+    if (stage == 3 && half == 0) continue;               /* only the final kinetic half */
     for (int axis = 1; axis > aK; --axis) {
       lg_pass_init(&p, &s, prec, psi, psi, axis);
       if (yscale) {
@@ -440,6 +460,12 @@ static void fused_step(wf *gwf, const step_args *sa) {
           p.flags |= HGRID_PF_MID_POT;
           p.pot = sa->pot;
         }
+        if ((stage == 1 || stage == 2) && sa->want_post_norm) {
+          /* the step's output is this pass's: sum |S|^2 of the stored transform (n2 times the
+           * real-space sum, which the finalize multiplies in) */
+          p.flags |= HGRID_PF_POST_RED;
+          p.partials = part9;
+        }
         lg_run_pass(&p);
       } else {
         lg_pass_init(&p, &s, prec, psi, psi, 2);
@@ -468,7 +494,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
     }
     lg_run_pass(&p);
   } else {
-    nbpost = nb1;
+    nbpost = (stage == 1 || stage == 2) ? nb2 : nb1;
   }
   if (sa->want_post_norm) {
     hgrid_reduce_job j = {part9, nbpost, HGRID_RED_POST, 1, d->dscal + GS_RED + HGRID_RED_POST};
@@ -486,6 +512,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
   d->valid = GV_DEV;
   d->pending = sa->renorm_target > 0.0;
   d->zpend = defer;
+  d->kpend = stage == 1 || stage == 2;
 }
 
 /* the propagator branches of grid_wf_propagate (reference src/grid_wf.c:390-525) */
@@ -718,7 +745,13 @@ void grid_wf_gpe_propagate(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, c
 
 /* one fused GPE step; without want_energy the energy reductions are left out, and with
  * g = 0 a deferred 3-D state is not read back at all before the transforms */
+static void gpe_stage(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize, int want_energy, int stage);
+
 static void gpe_step(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize, int want_energy) {
+  gpe_stage(gwf, vext, g_nl, time, normalize, want_energy, 0);
+}
+
+static void gpe_stage(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char normalize, int want_energy, int stage) {
   cgrid *g = gwf->grid;
   check_split_propagator(gwf, "grid_wf_gpe_propagate", 0);
   if (g->omega != 0.0) lg_fatal("omega != 0.0 allowed only with CN.");
@@ -744,9 +777,10 @@ static void gpe_step(wf *gwf, rgrid *vext, REAL g_nl, REAL complex time, char no
   sa.want_energy = want_energy;
   sa.want_post_norm = normalize ? 1 : 0;
   sa.renorm_target = normalize ? gwf->norm : 0.0;
+  sa.stage = stage;
   fused_step(gwf, &sa);
   grid_dev *d = g->dev;
-  d->have_gpe_energy = want_energy;
+  if (stage <= 1) d->have_gpe_energy = want_energy;
   g_last_g = g_nl;
 }
 
@@ -778,6 +812,22 @@ REAL grid_wf_gpe_energy(wf *gwf, rgrid *vext, REAL g_nl) {
 static int g_use_graphs = 1;
 
 void grid_hip_graphs(int enable) { g_use_graphs = enable ? 1 : 0; }
+
+static int g_merge_kinetic = 1;
+
+void grid_hip_merged_relaxation(int enable) { g_merge_kinetic = enable ? 1 : 0; }
+
+/* Linear relaxation (g = 0) of a 3-D grid with the plain 2nd-order propagator runs its batches
+ * with merged kinetic half-steps: the last K(t/2) of one step and the first of the next are one
+ * K(t), so a step costs the four passes y, x (K), y, z (V) instead of seven.  The sequence of
+ * states at the batch boundaries, where energies, snapshots and the stopping rule look, is that
+ * of whole steps (K K = K(t) in the phases). */
+static int merged_batches(const wf *gwf, REAL g_nl) {
+  const cgrid *g = gwf->grid;
+  lg_shape s;
+  lg_squeeze(g->nx, g->ny, g->nz, &s);
+  return g_merge_kinetic && g_nl == 0.0 && s.m == 3 && gwf->propagator == WF_2ND_ORDER_FFT;
+}
 
 /* Vector extrapolation of the relaxation (used when stride > 0).  Snapshots psi_k are taken
  * every `stride` steps.  With the differences d1 = psi_k - psi_{k-1} and d2 = psi_{k+1} - psi_k,
@@ -856,7 +906,7 @@ static int extrap_snapshot(wf *gwf, extrap_state *x, double *r_out) {
  * reductions, a precision change), so a graph is replayed only while every one of them is
  * unchanged, and is captured again otherwise. */
 typedef struct {
-  const void *p[14];
+  const void *p[17];
   unsigned long partials_gen;
 } graph_key;
 
@@ -876,6 +926,7 @@ static void graph_key_of(wf *gwf, rgrid *vext, const double *hist, graph_key *k)
   k->p[i++] = d->kin.kc[0];
   k->p[i++] = d->kin.kc[1];
   k->p[i++] = d->kin.kc[2];
+  for (int a = 0; a < 3; ++a) k->p[i++] = d->kin2.t[a];
   k->partials_gen = lg_partials_generation();
 }
 
@@ -892,6 +943,7 @@ static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
   lg_hip(hgrid_malloc((void **)&hist, sizeof(double) * HGRID_RED_NCHAN), "hipMalloc (energy history)");
   double host[HGRID_RED_NCHAN];
   double e = NAN;
+  const int merge = merged_batches(gwf, g_nl);
   INT n = 0, n0 = 0; /* n0: step at which the stopping rule (re)started */
   int converged = 0;
   lg_relax_state rs;
@@ -924,9 +976,10 @@ static INT relax_impl(wf *gwf, rgrid *vext, REAL g_nl, REAL dt, REAL tol, INT ma
       for (INT b = 0; b < nb; ++b) {
         /* the rule's checkpoints fall on multiples of its strides, all multiples of 16 from
          * its start, so only the first step of a batch computes its input state's energy */
-        gpe_step(gwf, vext, g_nl, -I * dt, 1, b == 0);
+        gpe_stage(gwf, vext, g_nl, -I * dt, 1, b == 0, merge ? (b == 0 ? 1 : 2) : 0);
         if (b == 0) lg_hip(hgrid_copy_f64(hist, d->dscal + GS_RED, HGRID_RED_NCHAN), "copy (energy history)");
       }
+      if (merge) gpe_stage(gwf, vext, g_nl, -I * dt, 1, 0, 3);
       if (capture) {
         lg_hip(hgrid_graph_end(&graph), "hipStreamEndCapture (relaxation)");
         graph_key_of(gwf, vext, hist, &key);

@@ -45,6 +45,9 @@ typedef struct grid_dev {
   void *work;              /* device-only scratch of the same size (kinetic energy etc.) */
   double *rho;             /* device density scratch for the GPE step */
   ktab_cache kin;          /* kinetic tables for the last (time, mass, frame) */
+  ktab_cache kin2;         /* the full-step tables of a merged relaxation step (twice that time) */
+  int kpend;               /* inside a merged relaxation batch: the array holds the state before
+                              the last half kinetic factor of the step (never seen by callers) */
   ktab_cache kin_energy;   /* |k|^2 tables for energy reductions */
   double last_energy_parts[4];
   int have_gpe_energy;
@@ -94,6 +97,7 @@ void lg_fft_inverse(void *in, void *out, int prec, const lg_shape *s, double pos
 /* kinetic tables: exp(tm (k - k0)^2) per physical axis; `fold` multiplies the table of the
  * innermost non-trivial axis (and absorbs trivial axes' constant factors). */
 ktab_cache *lg_kinetic_tables(cgrid *g, double tm_re, double tm_im, double fold);
+ktab_cache *lg_kinetic_tables_in(cgrid *g, ktab_cache *k, double tm_re, double tm_im, double fold);
 ktab_cache *lg_k2_tables(cgrid *g);
 /* sum over an array of (k2x[i] + k2y[j] + k2z[k]) |psi|^2 -> device out */
 int lg_sum_k2norm2(double *out, const void *psi, int prec, long nx, long ny, long nz, const double *k2x,

@@ -145,6 +145,10 @@ def test_vortex_ring_512_fp32_vs_fp64(lg):
     print(f"vortex ring 512^3 after {prm['steps']} steps: fp32 vs fp64 rel L2 {err:.3e}; norm drift fp64 "
           f"{d64['norm_drift']:.2e} fp32 {d32['norm_drift']:.2e}; energy drift fp64 {d64['e_end'] - d64['e0']:.3e} "
           f"fp32 {d32['e_end'] - d32['e0']:.3e}; E0 {d64['e0']:.12f}")
+    # measured on MI355X: rel L2 1.26e-4, FP32 norm drift 2.4e-4, FP32 energy drift -1.95e-4
+    # (FP64: 2e-13 and 5.8e-7); the bounds leave a factor of two
     assert d64["norm_drift"] < 1e-11
-    assert err < 1e-3
-    assert d32["norm_drift"] < 1e-3
+    assert abs(d64["e_end"] - d64["e0"]) < 2e-6
+    assert err < 2.5e-4
+    assert d32["norm_drift"] < 5e-4
+    assert abs(d32["e_end"] - d32["e0"]) < 4e-4
