@@ -116,7 +116,8 @@ typedef struct hgrid_pass {
   /* kinetic tables (precision prec): factor for the element index along axis, and for the two
    * other squeezed coordinates c1, c2 of the pencil (see hgrid_pass_coords) */
   const void *ktab_a, *ktab_1, *ktab_2;
-  const double *k2_a, *k2_1, *k2_2;   /* |k|^2 per axis for MID_KRED (no k0 shift) */
+  const double *k2_a, *k2_1, *k2_2;   /* |k|^2 per axis for MID_KRED (no k0 shift); the kernels
+                                       * form the pass axis's own entries from k2_dk */
   const double *kc_a, *kc_1, *kc_2;   /* (k - k0)^2 per axis for KCUT */
   double kcut2;                       /* KCUT threshold kmax^2 */
   /* potential */
@@ -128,6 +129,8 @@ typedef struct hgrid_pass {
   double cons;           /* constant added to the potential */
   double *rho_out;       /* PRE_RHO destination (FP64, dense) */
   double *partials;      /* HGRID_RED_NCHAN * nblocks doubles (see hgrid_pass_blocks) */
+  double k2_dk;          /* MID_KRED: k spacing 2 pi / (step L) of the pass axis; element i has
+                          * k = i dk (i <= L/2) or (i - L) dk and |k|^2 = k * k, as in k2_a */
 } hgrid_pass;
 
 /* Number of workgroups a pass launches (size of each partials channel). */

@@ -168,6 +168,7 @@ class hgrid_pass(C.Structure):
         ("cons", C.c_double),
         ("rho_out", C.c_void_p),
         ("partials", C.c_void_p),
+        ("k2_dk", C.c_double),
     ]
 
 

@@ -111,6 +111,17 @@ template <bool WAVE> __device__ __forceinline__ void pencil_sync() {
   }
 }
 
+// The twiddle table W_L^m, m < L, as a buffer resource whose address is hidden from the
+// optimiser: the twiddles of a second transform in the same pass are then reloaded (L1 hits)
+// instead of being kept live across the mid-pass work, and, as buffer loads, they are counted
+// by vmcnt alone, so waiting for them does not also wait for the LDS exchange (flat loads of a
+// laundered pointer count on both)
+template <typename C, int L> __device__ __forceinline__ __amdgpu_buffer_rsrc_t twiddle_rsrc(const C *tw) {
+  unsigned long long b = (unsigned long long)tw;
+  __asm__ volatile("" : "+s"(b));
+  return make_rsrc((const void *)b, (unsigned long)L * sizeof(C));
+}
+
 // One 512-point transform of the pencil owned by (q, t), 64 threads of 8 values, thread t holding
 // x[t + 64 j].  With n = t1 + 8 t2 + 64 j and k = k1 + 8 k2 + 64 k3:
 //   stage 1: 8-point DFTs over j, twiddle W_512^{t k1}        (thread t = t1 + 8 t2)
@@ -127,11 +138,10 @@ __device__ __forceinline__ void fft_core3(C (&v)[8], int t, int q, bool inverse,
 #pragma unroll
     for (int j = 0; j < 8; ++j) v[j] = cconj(v[j]);
   }
-  const C *twl = tw;
-  __asm__ volatile("" : "+s"(twl));
+  const auto twr = twiddle_rsrc<C, 512>(tw);
   dft<8>(v);
 #pragma unroll
-  for (int k1 = 1; k1 < 8; ++k1) v[k1] = cmul(v[k1], twl[t * k1]);
+  for (int k1 = 1; k1 < 8; ++k1) v[k1] = cmul(v[k1], bload<C>(twr, (unsigned)(t * k1) * (unsigned)sizeof(C), 0));
   pencil_sync<CONTIG>();  // the previous transform's exchange 2 may still be reading
 #pragma unroll
   for (int k1 = 0; k1 < 8; ++k1) lds[x1_index(k1, t >> 3, t & 7)] = v[k1];
@@ -141,7 +151,7 @@ __device__ __forceinline__ void fft_core3(C (&v)[8], int t, int q, bool inverse,
   for (int t2 = 0; t2 < 8; ++t2) v[t2] = lds[x1_index(b, t2, a)];
   dft<8>(v);
 #pragma unroll
-  for (int k2 = 1; k2 < 8; ++k2) v[k2] = cmul(v[k2], twl[8 * a * k2]);
+  for (int k2 = 1; k2 < 8; ++k2) v[k2] = cmul(v[k2], bload<C>(twr, (unsigned)(8 * a * k2) * (unsigned)sizeof(C), 0));
   pencil_sync<CONTIG>();  // exchange 1's reads of this buffer are done
 #pragma unroll
   for (int k2 = 0; k2 < 8; ++k2) lds[x2_index(b, k2, a)] = v[k2];
@@ -176,12 +186,9 @@ __device__ __forceinline__ void fft_core2(C (&v)[(Split<L, CONTIG>::R)], int t, 
   }
   dft<R>(v);
   if constexpr (TPP > 1) {
-    // Hide the table address from the optimiser so the twiddles of a second transform in the
-    // same pass are reloaded (L1 hits) instead of being kept live across the mid-pass work.
-    const C *twl = tw;
-    __asm__ volatile("" : "+s"(twl));
+    const auto twr = twiddle_rsrc<C, L>(tw);
 #pragma unroll
-    for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], twl[t * k1]);
+    for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], bload<C>(twr, (unsigned)(t * k1) * (unsigned)sizeof(C), 0));
     constexpr int A = R / TPP;
     // real parts.  A contiguous pencil's TPP threads are consecutive lanes of one wave and its
     // exchange region is its own, so ordering that wave's LDS accesses is enough there.
@@ -213,6 +220,15 @@ __device__ __forceinline__ void fft_core2(C (&v)[(Split<L, CONTIG>::R)], int t, 
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = cconj(v[j]);
   }
+}
+
+// |k|^2 of element i of an L-point axis with k spacing dk: k = i dk (i <= L/2) or (i - L) dk,
+// squared with its own rounding (no contraction into the sum that uses it), as the host tables
+// of |k|^2 are formed
+template <int L> __device__ __forceinline__ double k2_of(int i, double dk) {
+#pragma clang fp contract(off)
+  const double k = (double)(i <= L / 2 ? i : i - L) * dk;
+  return k * k;
 }
 
 // ---- one L-point transform of the pencil owned by (q, t) ----------------------------------
@@ -267,6 +283,10 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
   constexpr unsigned CS = sizeof(S);  // bytes per stored element
   __shared__ __align__(16) Rl lds[LDS_N];
   __shared__ double red_lds[NT / 64][HGRID_RED_NCHAN];
+  // kinetic pass on a strided axis: the axis's factor table staged in LDS (the transform's first
+  // workgroup barrier publishes it)
+  constexpr bool KT_LDS = KIND == KIND_KIN && !CONTIG;
+  __shared__ __align__(16) double2 kt_lds[KT_LDS ? L : 1];
 
   const int tid = threadIdx.x;
   int q, t;
@@ -324,6 +344,10 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
   const unsigned fl = a.flags;
   const C *__restrict__ tw = (const C *)a.tw;
   const auto rin = make_rsrc(a.in, (unsigned long)ntot * CS);
+  if constexpr (KT_LDS) {
+    const double2 *ka = (const double2 *)a.ktab_a;
+    for (int i = tid; i < L; i += NT) kt_lds[i] = ka[i];
+  }
 
   C v[R];
 #pragma unroll
@@ -378,16 +402,19 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
 
   if constexpr (KIND == KIND_FFT || KIND == KIND_KIN) {
     if (fl & HGRID_PF_MID_KRED) {
+      // |k|^2 of the pass axis formed in registers, bit for bit the table entry: a table load
+      // per element, issued among the 32 live values of a 512-point pencil, came out serialised
+      // (one wait per load; 0.75 -> 0.90 ms for the FP32 512^3 kinetic pass)
       const double kk = a.k2_1[c1] + a.k2_2[c2];
 #pragma unroll
-      for (int j = 0; j < R; ++j) acc[HGRID_RED_KIN] += (kk + a.k2_a[t + TPP * j]) * cnorm2(v[j]);
+      for (int j = 0; j < R; ++j) acc[HGRID_RED_KIN] += (kk + k2_of<L>(t + TPP * j, a.k2_dk)) * cnorm2(v[j]);
     }
   }
   if constexpr (KIND == KIND_KIN) {
     // FP64 tables whatever the storage precision; the factor is formed in FP64.  Real
     // factors (imaginary time) multiply in FP64 and only the product is rounded, so FP32
     // storage sees no fixed per-element error in K; complex ones are rounded once to C.
-    const double2 *ka = (const double2 *)a.ktab_a;
+    const double2 *ka = KT_LDS ? (const double2 *)kt_lds : (const double2 *)a.ktab_a;
     const double2 f = cmul(((const double2 *)a.ktab_1)[c1], ((const double2 *)a.ktab_2)[c2]);
     if (fl & HGRID_PF_KIN_REAL) {
       if constexpr (std::is_same<C, float2>::value) {

@@ -261,6 +261,9 @@ static void set_k2(hgrid_pass *p, const ktab_cache *k, const lg_shape *s, const 
   p->k2_a = k2_sq(k, s, g, p->axis);
   p->k2_1 = k2_sq(k, s, g, a1);
   p->k2_2 = k2_sq(k, s, g, a2);
+  /* the pass axis's entries are formed in the kernel: the same expression as lg_k2_tables */
+  const long L = p->axis == 0 ? p->n0 : p->axis == 1 ? p->n1 : p->n2;
+  p->k2_dk = 2.0 * M_PI / (g->step * (double)L);
 }
 
 static void set_kc(hgrid_pass *p, const ktab_cache *k, const lg_shape *s, const cgrid *g, double kcut2) {
