@@ -35,10 +35,15 @@ extern "C" int hgrid_fft_pass(const hgrid_pass *p) {
   else if (p->flags & HGRID_PF_MID_POT) kind = lg::KIND_POT;
   else if (p->flags & HGRID_PF_MID_GPE) kind = lg::KIND_GPE;
   else kind = lg::KIND_FFT;
+  // the read-only start-of-step pass: inverse transform, then density / V_eff / reductions
+  constexpr unsigned PRE_ONLY = HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE;
+  constexpr unsigned PRE_KEEP = HGRID_PF_PRESCALE | HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF;
+  if (kind == lg::KIND_FFT && p->axis == 2 && (p->flags & PRE_ONLY) == PRE_ONLY && !(p->flags & ~(PRE_ONLY | PRE_KEEP)))
+    kind = lg::KIND_PRE;
   const unsigned skip = p->flags & (HGRID_PF_POT_NOFFT1 | HGRID_PF_POT_NOFFT2);
   if (skip && (kind != lg::KIND_POT || skip == (HGRID_PF_POT_NOFFT1 | HGRID_PF_POT_NOFFT2))) return -1;
-  if (kind == lg::KIND_FFT && p->nfft != 1) return -1;
-  if (kind != lg::KIND_FFT && p->nfft != (skip ? 1 : 2)) return -1;
+  if ((kind == lg::KIND_FFT || kind == lg::KIND_PRE) && p->nfft != 1) return -1;
+  if (kind != lg::KIND_FFT && kind != lg::KIND_PRE && p->nfft != (skip ? 1 : 2)) return -1;
   if (p->prec == HGRID_C128) return lg::launch_f64(p, L, kind);
   if (p->prec == HGRID_C64) return lg::launch_f32(p, L, kind);
   if (p->prec == HGRID_C64X) return lg::launch_f32s_f64(p, L, kind);

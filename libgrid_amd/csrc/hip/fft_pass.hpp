@@ -23,6 +23,8 @@
 //   KIND_KIN : forward, kinetic factor (optionally sum k^2|psi_k|^2 first), inverse
 //   KIND_POT : inverse, mid * exp(c (cons + V)) with complex V, forward
 //   KIND_GPE : inverse, mid * exp(c (cons + vext + g rho)), forward
+//   KIND_PRE : one transform on a contiguous axis, then |psi|^2 / V_eff / reductions of the
+//              result, nothing stored (the start-of-step pass over a state kept transformed)
 // Inverse transforms use conj(DFT(conj(x))), so one set of butterflies serves both signs.
 // Reference operators this replaces: cgrid_fft / cgrid_inverse_fft (src/cgrid.c:1822-1846,
 // src/cgrid_fft.c:101-146), the kinetic phase (src/grid_wf_fft.c:141-193), the potential
@@ -38,7 +40,7 @@
 
 namespace lg {
 
-enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3 };
+enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3, KIND_PRE = 4 };
 
 
 // Threads per pencil (TPP) and values per thread (R) for each supported length L = R * TPP:
@@ -341,7 +343,9 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
   // element j of this thread: flat index base + (t + TPP*j)*astride
   const unsigned e0 = (unsigned)base + (unsigned)t * astride;  // flat index of element 0
   const unsigned estep = (unsigned)TPP * astride;              // flat index step per j
-  const unsigned fl = a.flags;
+  // KIND_PRE: a KIND_FFT pass whose other flags are known to be clear, compiled without them
+  constexpr unsigned PRE_KEEP = HGRID_PF_PRESCALE | HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF;
+  const unsigned fl = KIND == KIND_PRE ? (a.flags & PRE_KEEP) | HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE : a.flags;
   const C *__restrict__ tw = (const C *)a.tw;
   const auto rin = make_rsrc(a.in, (unsigned long)ntot * CS);
   if constexpr (KT_LDS) {
@@ -355,7 +359,7 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
 
   double acc[HGRID_RED_NCHAN] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
-  if constexpr (KIND == KIND_FFT || KIND == KIND_POT) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_POT || KIND == KIND_PRE) {
     if (fl & HGRID_PF_PRESCALE) {
       const C s = mk<C>((Rl)a.scale[0], (Rl)a.scale[1]);
 #pragma unroll
@@ -365,14 +369,23 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
   // density / V_eff / input reductions of the real-space data: before the transform, or after
   // it with PRE_AFTER (the data is then stored transformed along this axis and the pass
   // inverse-transforms it in registers); element j keeps flat index e0 + j * estep either way
+  const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
+  // the start-of-step pass loads vext with the state, before its transform, so the two reads
+  // share one memory latency (256^3 FP64: 112 -> 102 us at 164 instead of 128 VGPRs)
+  double vxp[KIND == KIND_PRE ? R : 1];
+  if constexpr (KIND == KIND_PRE) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) vxp[j] = (a.vext && active) ? bload_f64(rvx, (e0 + (unsigned)j * estep) * 8u, 0) : 0.0;
+  }
   auto pre_ops = [&]() {
     const auto rrho = make_rsrc(a.rho_out, (unsigned long)ntot * 8);
-    const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const unsigned e = e0 + (unsigned)j * estep;
       const double d = cnorm2(v[j]);
-      const double vx = (a.vext && active) ? bload_f64(rvx, e * 8u, 0) : 0.0;
+      double vx;
+      if constexpr (KIND == KIND_PRE) vx = vxp[j];
+      else vx = (a.vext && active) ? bload_f64(rvx, e * 8u, 0) : 0.0;
       if (active) {
         if (fl & HGRID_PF_PRE_VEFF) bstore_f64(vx + a.g * d, rrho, e * 8u, 0);
         else if (fl & HGRID_PF_PRE_RHO) bstore_f64(d, rrho, e * 8u, 0);
@@ -396,7 +409,7 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
     fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
   }
 
-  if constexpr (KIND == KIND_FFT) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_PRE) {
     if ((fl & PRE_ANY) && (fl & HGRID_PF_PRE_AFTER)) pre_ops();
   }
 
@@ -496,7 +509,7 @@ __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(cons
 
   if constexpr (KIND == KIND_POT) {
     if (!(fl & HGRID_PF_POT_NOFFT2)) fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
-  } else if constexpr (KIND != KIND_FFT) {
+  } else if constexpr (KIND != KIND_FFT && KIND != KIND_PRE) {
     fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
   }
 
@@ -572,6 +585,7 @@ template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long 
     case KIND_KIN: return contig ? launch_len<C, true, KIND_KIN, S>(p, L) : launch_len<C, false, KIND_KIN, S>(p, L);
     case KIND_POT: return contig ? launch_len<C, true, KIND_POT, S>(p, L) : -1;
     case KIND_GPE: return contig ? launch_len<C, true, KIND_GPE, S>(p, L) : -1;
+    case KIND_PRE: return contig ? launch_len<C, true, KIND_PRE, S>(p, L) : -1;
     default: return -1;
   }
 }
