@@ -276,8 +276,17 @@ template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, do
 
 // C is the arithmetic type, S the storage type of psi (and of a complex potential): S = C, or
 // S = float2 with C = double2 ("FP32 storage, FP64 arithmetic": only the stores round).
+// Minimum waves per SIMD asked of the register allocator, for kernels that reach it without
+// scratch spills and run faster there (measured per kernel; 1 = the allocator's own choice).
+// FP64 256-point kinetic pass on a strided axis: 170 -> 168 VGPRs, 2 -> 3 waves, 100 -> 90 us
+// at 256^3.
+template <typename C, int L, bool CONTIG, int KIND> constexpr int waves_hint() {
+  return (KIND == KIND_KIN && !CONTIG && L == 256 && sizeof(C) == 16) ? 3 : 1;
+}
+
 template <typename C, int L, bool CONTIG, int KIND, typename S = C>
-__global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT)) fft_pass_kernel(const hgrid_pass a) {
+__global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT))
+__attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_kernel(const hgrid_pass a) {
   using Rl = typename real_of<C>::type;
   using G = Geom<C, L, CONTIG>;
   constexpr int TPP = Split<L, CONTIG>::TPP, R = Split<L, CONTIG>::R, NT = G::NT, P = G::P;
