@@ -31,9 +31,9 @@ extern "C" int hgrid_fft_pass(const hgrid_pass *p) {
   const long esize = p->prec == HGRID_C128 ? 16 : 8;
   if (ntot * esize > 0xffffffffL) return -1;
   int kind;
-  if (p->flags & HGRID_PF_MID_KIN) kind = lg::KIND_KIN;
+  if (p->flags & HGRID_PF_MID_KIN) kind = (p->flags & HGRID_PF_KIN_REAL) ? lg::KIND_KIR : lg::KIND_KIN;
   else if (p->flags & HGRID_PF_MID_POT) kind = lg::KIND_POT;
-  else if (p->flags & HGRID_PF_MID_GPE) kind = lg::KIND_GPE;
+  else if (p->flags & HGRID_PF_MID_GPE) kind = p->c_im == 0.0 ? lg::KIND_GPR : lg::KIND_GPE;
   else kind = lg::KIND_FFT;
   // the read-only start-of-step pass: inverse transform, then density / V_eff / reductions
   constexpr unsigned PRE_ONLY = HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE;

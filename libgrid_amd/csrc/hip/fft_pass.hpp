@@ -20,9 +20,14 @@
 // Pass kinds (compile time, so each kernel carries only the code it runs):
 //   KIND_FFT : one transform; optional pending scale on load, |psi|^2 / reductions on the
 //              input, real post factor and |psi|^2 reduction on the output
-//   KIND_KIN : forward, kinetic factor (optionally sum k^2|psi_k|^2 first), inverse
+//   KIND_KIN : forward, complex kinetic factor (optionally sum k^2|psi_k|^2 first), inverse
+//   KIND_KIR : KIND_KIN with real factors (imaginary time).  Separate kinds because a kernel
+//              holding both factor paths pays the registers of both (strided 512-point FP32
+//              kinetic pass: 254 VGPRs with both, 166 with the complex path alone)
 //   KIND_POT : inverse, mid * exp(c (cons + V)) with complex V, forward
 //   KIND_GPE : inverse, mid * exp(c (cons + vext + g rho)), forward
+//   KIND_GPR : KIND_GPE with real c (imaginary time) only: without the complex-phase code the
+//              256-point FP64 pass fits 3 waves per SIMD (166 instead of 224 VGPRs)
 //   KIND_PRE : one transform on a contiguous axis, then |psi|^2 / V_eff / reductions of the
 //              result, nothing stored (the start-of-step pass over a state kept transformed)
 // Inverse transforms use conj(DFT(conj(x))), so one set of butterflies serves both signs.
@@ -40,7 +45,7 @@
 
 namespace lg {
 
-enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3, KIND_PRE = 4 };
+enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3, KIND_PRE = 4, KIND_GPR = 5, KIND_KIR = 6 };
 
 
 // Threads per pencil (TPP) and values per thread (R) for each supported length L = R * TPP:
@@ -276,17 +281,23 @@ template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, do
 
 // C is the arithmetic type, S the storage type of psi (and of a complex potential): S = C, or
 // S = float2 with C = double2 ("FP32 storage, FP64 arithmetic": only the stores round).
-// Minimum waves per SIMD asked of the register allocator, for kernels that reach it without
-// scratch spills and run faster there (measured per kernel; 1 = the allocator's own choice).
-// FP64 256-point kinetic pass on a strided axis: 170 -> 168 VGPRs, 2 -> 3 waves, 100 -> 90 us
-// at 256^3.
+// Waves per SIMD asked of the register allocator (minimum, maximum), per kernel, where measured
+// faster; (1, 8) leaves the allocator its own choice.  Strided kinetic passes in FP64: at 256
+// points exactly 3 (158 VGPRs: 100 -> 92 us at 256^3; at 4 waves the allocator's 128-VGPR
+// schedule ran 111 us), at 512 points at least 2 (256 VGPRs instead of 256 + AGPR spills and 1
+// wave: 1.43 -> 1.29 ms at 512^3).  Hints that forced scratch spills (start-of-step pass at 4,
+// GPE pass at 3) were slower and are not used.
 template <typename C, int L, bool CONTIG, int KIND> constexpr int waves_hint() {
-  return (KIND == KIND_KIN && !CONTIG && L == 256 && sizeof(C) == 16) ? 3 : 1;
+  if ((KIND == KIND_KIN || KIND == KIND_KIR) && !CONTIG && L == 512 && sizeof(C) == 16) return 2;
+  return ((KIND == KIND_KIN || KIND == KIND_KIR) && !CONTIG && L == 256 && sizeof(C) == 16) ? 3 : 1;
+}
+template <typename C, int L, bool CONTIG, int KIND> constexpr int waves_cap() {
+  return ((KIND == KIND_KIN || KIND == KIND_KIR) && !CONTIG && L == 256 && sizeof(C) == 16) ? 3 : 8;
 }
 
 template <typename C, int L, bool CONTIG, int KIND, typename S = C>
 __global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT))
-__attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_kernel(const hgrid_pass a) {
+__attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C, L, CONTIG, KIND>()))) fft_pass_kernel(const hgrid_pass a) {
   using Rl = typename real_of<C>::type;
   using G = Geom<C, L, CONTIG>;
   constexpr int TPP = Split<L, CONTIG>::TPP, R = Split<L, CONTIG>::R, NT = G::NT, P = G::P;
@@ -296,7 +307,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_
   __shared__ double red_lds[NT / 64][HGRID_RED_NCHAN];
   // kinetic pass on a strided axis: the axis's factor table staged in LDS (the transform's first
   // workgroup barrier publishes it)
-  constexpr bool KT_LDS = KIND == KIND_KIN && !CONTIG;
+  constexpr bool KT_LDS = (KIND == KIND_KIN || KIND == KIND_KIR) && !CONTIG;
   __shared__ __align__(16) double2 kt_lds[KT_LDS ? L : 1];
 
   const int tid = threadIdx.x;
@@ -422,7 +433,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_
     if ((fl & PRE_ANY) && (fl & HGRID_PF_PRE_AFTER)) pre_ops();
   }
 
-  if constexpr (KIND == KIND_FFT || KIND == KIND_KIN) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_KIN || KIND == KIND_KIR) {
     if (fl & HGRID_PF_MID_KRED) {
       // |k|^2 of the pass axis formed in registers, bit for bit the table entry: a table load
       // per element, issued among the 32 live values of a 512-point pencil, came out serialised
@@ -432,13 +443,13 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_
       for (int j = 0; j < R; ++j) acc[HGRID_RED_KIN] += (kk + k2_of<L>(t + TPP * j, a.k2_dk)) * cnorm2(v[j]);
     }
   }
-  if constexpr (KIND == KIND_KIN) {
+  if constexpr (KIND == KIND_KIN || KIND == KIND_KIR) {
     // FP64 tables whatever the storage precision; the factor is formed in FP64.  Real
     // factors (imaginary time) multiply in FP64 and only the product is rounded, so FP32
     // storage sees no fixed per-element error in K; complex ones are rounded once to C.
     const double2 *ka = KT_LDS ? (const double2 *)kt_lds : (const double2 *)a.ktab_a;
     const double2 f = cmul(((const double2 *)a.ktab_1)[c1], ((const double2 *)a.ktab_2)[c2]);
-    if (fl & HGRID_PF_KIN_REAL) {
+    if constexpr (KIND == KIND_KIR) {
       if constexpr (std::is_same<C, float2>::value) {
         // FP32 data: groups of four behind scheduling fences keep few FP64 factors live
 #pragma unroll
@@ -470,7 +481,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_
       for (int j = 0; j < R; ++j)
         if ((a.kc_a[t + TPP * j] + kc1) + kc2 >= a.kcut2) v[j] = mk<C>(0, 0);
     }
-  } else if constexpr (KIND == KIND_POT || KIND == KIND_GPE) {
+  } else if constexpr (KIND == KIND_POT || KIND == KIND_GPE || KIND == KIND_GPR) {
     const Rl mid = (Rl)a.mid;
     const auto rpot = make_rsrc(a.pot, (unsigned long)ntot * CS);
     const auto rvx = make_rsrc(a.vext, a.vext ? (unsigned long)ntot * 8 : 0);
@@ -505,11 +516,11 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_
 #pragma unroll
         for (int j = 0; j < R; ++j) vr[j] = a.cons + bload_f64(rrho, (e0 + (unsigned)j * estep) * 8u, 0);
       }
-      if (a.c_im == 0.0) {
+      if (KIND == KIND_GPR || a.c_im == 0.0) {
         // imaginary time: a real decay factor exp(c_re * V)
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = cscale_d(v[j], (double)mid * real_decay(a.c_re * vr[j]));
-      } else {
+      } else if constexpr (KIND != KIND_GPR) {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, vr[j], 0.0));
       }
@@ -529,7 +540,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>()))) fft_pass_
       for (int j = 0; j < R; ++j) v[j] = cscale(v[j], post);
     }
   }
-  if constexpr (KIND == KIND_FFT || KIND == KIND_GPE) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_GPE || KIND == KIND_GPR) {
     // the stored values (a merged relaxation step renormalises from the output of its potential
     // pass; kinetic and complex-potential passes never reduce their output, and the accumulation
     // would cost the strided 512-point kinetic pass its second wave per SIMD)
@@ -592,8 +603,10 @@ template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long 
   switch (kind) {
     case KIND_FFT: return contig ? launch_len<C, true, KIND_FFT, S>(p, L) : launch_len<C, false, KIND_FFT, S>(p, L);
     case KIND_KIN: return contig ? launch_len<C, true, KIND_KIN, S>(p, L) : launch_len<C, false, KIND_KIN, S>(p, L);
+    case KIND_KIR: return contig ? launch_len<C, true, KIND_KIR, S>(p, L) : launch_len<C, false, KIND_KIR, S>(p, L);
     case KIND_POT: return contig ? launch_len<C, true, KIND_POT, S>(p, L) : -1;
     case KIND_GPE: return contig ? launch_len<C, true, KIND_GPE, S>(p, L) : -1;
+    case KIND_GPR: return contig ? launch_len<C, true, KIND_GPR, S>(p, L) : -1;
     case KIND_PRE: return contig ? launch_len<C, true, KIND_PRE, S>(p, L) : -1;
     default: return -1;
   }
