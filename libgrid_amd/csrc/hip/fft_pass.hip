@@ -15,8 +15,8 @@ extern "C" long hgrid_pass_blocks(int prec, long n0, long n1, long n2, int axis)
   const long L = axis_len(n0, n1, n2, axis);
   // the workgroup geometry follows the arithmetic type (double2 for FP32 storage with FP64 arithmetic)
   const bool contig = axis == 2;
-  const long P = prec == HGRID_C64 ? (contig ? lg::pencils_per_block<float2, true>(L) : lg::pencils_per_block<float2, false>(L))
-                                   : (contig ? lg::pencils_per_block<double2, true>(L) : lg::pencils_per_block<double2, false>(L));
+  const long P = prec == HGRID_C64 ? (contig ? lg::pencils_per_block<float2, true>(L, axis) : lg::pencils_per_block<float2, false>(L, axis))
+                                   : (contig ? lg::pencils_per_block<double2, true>(L, axis) : lg::pencils_per_block<double2, false>(L, axis));
   if (P <= 0) return 0;
   const long npencils = (n0 * n1 * n2) / L;
   return (npencils + P - 1) / P;

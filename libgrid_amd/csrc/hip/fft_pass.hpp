@@ -77,14 +77,20 @@ template <int L, bool CONTIG> struct Split {
 #define LG_FOR_EACH_LENGTH(X) \
   X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
 
-template <typename C, int L, bool CONTIG> struct Geom {
+// Wide tiles: on the outermost axis of a 512-point FP64 grid a row of a tile is 8 pencils x 16 B
+// = 128 B, one DRAM burst per 4 MiB-strided row.  There the workgroup takes 16 pencils (256 B
+// per row): x kinetic pass at 512^3 1.29 -> 0.97 ms.  The y axis (8 KiB row stride) keeps 8
+// pencils and two workgroups per CU: with 16 its pass went 0.77 -> 0.84 ms.
+template <typename C, int L, bool CONTIG> constexpr bool wide_capable() { return !CONTIG && L == 512 && sizeof(C) == 16; }
+
+template <typename C, int L, bool CONTIG, bool WIDE = false> struct Geom {
   static constexpr int TPP = Split<L, CONTIG>::TPP;
   static constexpr int R = Split<L, CONTIG>::R;
   static constexpr bool THREE = Split<L, CONTIG>::THREE;
   // workgroup size: two-stage lengths take 256 threads unless one real half of the LDS exchange
-  // exceeds 32 KiB.  Three-stage (contiguous only): pencils are one wave each, four per
-  // workgroup.
-  static constexpr int NT = THREE ? 256 : (TPP > 1 && (long)R * (long)sizeof(C) * 128 > 32768) ? 128 : 256;
+  // exceeds 32 KiB (and the tiles are not wide).  Three-stage (contiguous only): pencils are one
+  // wave each, four per workgroup.
+  static constexpr int NT = THREE || WIDE ? 256 : (TPP > 1 && (long)R * (long)sizeof(C) * 128 > 32768) ? 128 : 256;
   static constexpr int P = NT / TPP;  // pencils per workgroup
   // complex entries per pencil in the three-stage exchange buffer (512 + 4: consecutive pencils
   // start 4 entries apart in the LDS banks)
@@ -182,10 +188,10 @@ __device__ __forceinline__ int lds_index(int q, int k1, int t) {
 }
 
 // One L-point transform in two register stages (the decomposition at the top of this file).
-template <typename C, int L, bool CONTIG>
+template <typename C, int L, bool CONTIG, bool WIDE>
 __device__ __forceinline__ void fft_core2(C (&v)[(Split<L, CONTIG>::R)], int t, int q, bool inverse, const C *__restrict__ tw,
                                           typename real_of<C>::type *lds) {
-  constexpr int TPP = Split<L, CONTIG>::TPP, R = Split<L, CONTIG>::R, P = Geom<C, L, CONTIG>::P;
+  constexpr int TPP = Split<L, CONTIG>::TPP, R = Split<L, CONTIG>::R, P = Geom<C, L, CONTIG, WIDE>::P;
   static_assert(!CONTIG || 64 % TPP == 0, "a contiguous pencil's threads lie in one wave");
   if (inverse) {
 #pragma unroll
@@ -240,13 +246,13 @@ template <int L> __device__ __forceinline__ double k2_of(int i, double dk) {
 
 // ---- one L-point transform of the pencil owned by (q, t) ----------------------------------
 
-template <typename C, int L, bool CONTIG>
+template <typename C, int L, bool CONTIG, bool WIDE>
 __device__ __forceinline__ void fft_core(C (&v)[(Split<L, CONTIG>::R)], int t, int q, bool inverse, const C *__restrict__ tw,
                                          typename real_of<C>::type *lds) {
   if constexpr (Split<L, CONTIG>::THREE) {
     fft_core3<C, CONTIG>(v, t, q, inverse, tw, lds);
   } else {
-    fft_core2<C, L, CONTIG>(v, t, q, inverse, tw, lds);
+    fft_core2<C, L, CONTIG, WIDE>(v, t, q, inverse, tw, lds);
   }
 }
 
@@ -295,11 +301,11 @@ template <typename C, int L, bool CONTIG, int KIND> constexpr int waves_cap() {
   return ((KIND == KIND_KIN || KIND == KIND_KIR) && !CONTIG && L == 256 && sizeof(C) == 16) ? 3 : 8;
 }
 
-template <typename C, int L, bool CONTIG, int KIND, typename S = C>
-__global__ void __launch_bounds__((Geom<C, L, CONTIG>::NT))
+template <typename C, int L, bool CONTIG, int KIND, typename S = C, bool WIDE = false>
+__global__ void __launch_bounds__((Geom<C, L, CONTIG, WIDE>::NT))
 __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C, L, CONTIG, KIND>()))) fft_pass_kernel(const hgrid_pass a) {
   using Rl = typename real_of<C>::type;
-  using G = Geom<C, L, CONTIG>;
+  using G = Geom<C, L, CONTIG, WIDE>;
   constexpr int TPP = Split<L, CONTIG>::TPP, R = Split<L, CONTIG>::R, NT = G::NT, P = G::P;
   constexpr int LDS_N = CONTIG ? G::LDS_CONTIG : G::LDS_STRIDED;
   constexpr unsigned CS = sizeof(S);  // bytes per stored element
@@ -424,9 +430,9 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
 
   if constexpr (KIND == KIND_POT) {
     // the 4th-order step's outer V stages: potential factor before or after one transform
-    if (!(fl & HGRID_PF_POT_NOFFT1)) fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
+    if (!(fl & HGRID_PF_POT_NOFFT1)) fft_core<C, L, CONTIG, WIDE>(v, t, q, a.dir1 > 0, tw, lds);
   } else {
-    fft_core<C, L, CONTIG>(v, t, q, a.dir1 > 0, tw, lds);
+    fft_core<C, L, CONTIG, WIDE>(v, t, q, a.dir1 > 0, tw, lds);
   }
 
   if constexpr (KIND == KIND_FFT || KIND == KIND_PRE) {
@@ -528,9 +534,9 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   }
 
   if constexpr (KIND == KIND_POT) {
-    if (!(fl & HGRID_PF_POT_NOFFT2)) fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
+    if (!(fl & HGRID_PF_POT_NOFFT2)) fft_core<C, L, CONTIG, WIDE>(v, t, q, a.dir2 > 0, tw, lds);
   } else if constexpr (KIND != KIND_FFT && KIND != KIND_PRE) {
-    fft_core<C, L, CONTIG>(v, t, q, a.dir2 > 0, tw, lds);
+    fft_core<C, L, CONTIG, WIDE>(v, t, q, a.dir2 > 0, tw, lds);
   }
 
   if constexpr (KIND == KIND_FFT) {
@@ -579,13 +585,20 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   }
 }
 
-template <typename C, int L, bool CONTIG, int KIND, typename S> int launch(const hgrid_pass *p) {
-  constexpr int NT = Geom<C, L, CONTIG>::NT, P = Geom<C, L, CONTIG>::P;
+template <typename C, int L, bool CONTIG, int KIND, typename S, bool WIDE> int launch_geom(const hgrid_pass *p) {
+  constexpr int NT = Geom<C, L, CONTIG, WIDE>::NT, P = Geom<C, L, CONTIG, WIDE>::P;
   const long npencils = (p->n0 * p->n1 * p->n2) / L;
   const long nblocks = (npencils + P - 1) / P;
   if (nblocks <= 0) return 0;
-  hipLaunchKernelGGL((fft_pass_kernel<C, L, CONTIG, KIND, S>), dim3((unsigned)nblocks), dim3(NT), 0, stream(), *p);
+  hipLaunchKernelGGL((fft_pass_kernel<C, L, CONTIG, KIND, S, WIDE>), dim3((unsigned)nblocks), dim3(NT), 0, stream(), *p);
   return check_launch("fft_pass");
+}
+
+template <typename C, int L, bool CONTIG, int KIND, typename S> int launch(const hgrid_pass *p) {
+  if constexpr (wide_capable<C, L, CONTIG>()) {
+    if (p->axis == 0) return launch_geom<C, L, CONTIG, KIND, S, true>(p);
+  }
+  return launch_geom<C, L, CONTIG, KIND, S, false>(p);
 }
 
 template <typename C, bool CONTIG, int KIND, typename S> int launch_len(const hgrid_pass *p, long L) {
@@ -612,9 +625,10 @@ template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long 
   }
 }
 
-template <typename C, bool CONTIG> long pencils_per_block(long L) {
+// pencils per workgroup of a pass along `axis` (the geometry launch() picks)
+template <typename C, bool CONTIG> long pencils_per_block(long L, int axis) {
 #define LG_CASE(n) \
-  case n: return Geom<C, n, CONTIG>::P;
+  case n: return wide_capable<C, n, CONTIG>() && axis == 0 ? Geom<C, n, CONTIG, true>::P : Geom<C, n, CONTIG>::P;
   switch (L) {
     LG_FOR_EACH_LENGTH(LG_CASE)
     default: return 0;

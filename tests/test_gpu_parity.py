@@ -195,7 +195,10 @@ def test_scaled_inverse_and_lazy_multiply(lg, orc):
 
 
 @pytest.mark.parametrize("shape,k0,time", [((16, 32, 64), (0.0, 0.0, 0.0), 0.05), ((8, 8, 8), (0.3, -0.2, 0.1), 0.03 - 0.02j),
-                                           ((1, 64, 32), (0.0, 0.5, 0.0), -0.1j)])
+                                           ((1, 64, 32), (0.0, 0.5, 0.0), -0.1j),
+                                           # 512 points on the outer axis: 16-pencil workgroups,
+                                           # the last one partial (24 and 36 pencils)
+                                           ((512, 2, 12), (0.2, 0.0, 0.0), -0.1j), ((512, 3, 12), (0.0, 0.0, 0.0), 0.03)])
 def test_kinetic_phase(lg, orc, shape, k0, time):
     case(f"kinetic {shape} k0={k0} t={time}")
     step, mass = 0.3, 1.7
@@ -230,6 +233,8 @@ def test_potential_phase(lg, orc, tstep, cons):
     ((1, 1, 256), (0, 0, 0.2), 0.05, False),
     ((32, 16, 16), (0, 0, 0), 0.05, True),
     ((8, 512, 4), (0, 0, 0), 0.01, False),
+    ((512, 4, 6), (0.0, 0.1, 0.0), 0.01, False),
+    ((512, 6, 4), (0, 0, 0), -0.02j, False),
 ])
 def test_propagate_matches_oracle(lg, orc, shape, k0, time, nullpot):
     case(f"propagate {shape} k0={k0} t={time} nullpot={nullpot}")
