@@ -105,10 +105,12 @@ template <typename C, int L, bool CONTIG, bool WIDE = false> struct Geom {
 // Exchange 1 moves Y[t1 + 8 t2][k1]: written by thread t1 + 8 t2 (k1 = 0..7), read by thread
 // t1 + 8 k1 (t2 = 0..7); exchange 2 moves Z[t1, k1][k2]: written by thread t1 + 8 k1 (k2 = 0..7),
 // read by thread k1 + 8 k2 (t1 = 0..7).  The XOR terms permute entries within rows of 8 so that
-// the 64 lanes of a wave access spread over the LDS banks for every one of these patterns.
-__device__ __forceinline__ int x1_index(int k1, int t2, int t1) { return (k1 << 6) + ((t2 ^ (k1 & 1)) << 3) + t1; }
+// the lanes an LDS cycle serves (8 to 32 of them for 8- or 16-byte entries at 128 or 256 bytes
+// per cycle) hit distinct banks in all four patterns; an earlier permutation left 2-way
+// conflicts in exchange 2's reads (33.6 M conflict cycles per 512^3 FP64 GPE pass).
+__device__ __forceinline__ int x1_index(int k1, int t2, int t1) { return (k1 << 6) + ((t2 ^ (k1 & 3)) << 3) + t1; }
 __device__ __forceinline__ int x2_index(int k1, int k2, int t1) {
-  return (k2 << 6) + (k1 << 3) + (t1 ^ ((k2 + (k1 >> 1)) & 7));
+  return (k2 << 6) + (k1 << 3) + (t1 ^ ((k1 ^ (k2 & 3)) & 7));
 }
 
 // Ordering of a pencil's LDS exchange.  A contiguous three-stage pencil is exactly one wave,
