@@ -31,7 +31,11 @@ extern "C" int hgrid_fft_pass(const hgrid_pass *p) {
   const long esize = p->prec == HGRID_C128 ? 16 : 8;
   if (ntot * esize > 0xffffffffL) return -1;
   int kind;
-  if (p->flags & HGRID_PF_MID_KIN) kind = (p->flags & HGRID_PF_KIN_REAL) ? lg::KIND_KIR : lg::KIND_KIN;
+  if (p->flags & HGRID_PF_MID_KIN) {
+    kind = (p->flags & HGRID_PF_KIN_REAL) ? lg::KIND_KIR : lg::KIND_KIN;
+    // real factors on a strided FP32 512-point axis: the kernel with both factor paths
+    if (kind == lg::KIND_KIR && p->prec == HGRID_C64 && p->axis != 2 && L == 512) kind = lg::KIND_KIB;
+  }
   else if (p->flags & HGRID_PF_MID_POT) kind = lg::KIND_POT;
   else if (p->flags & HGRID_PF_MID_GPE) kind = p->c_im == 0.0 ? lg::KIND_GPR : lg::KIND_GPE;
   else kind = lg::KIND_FFT;

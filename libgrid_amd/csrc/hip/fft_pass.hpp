@@ -24,6 +24,9 @@
 //   KIND_KIR : KIND_KIN with real factors (imaginary time).  Separate kinds because a kernel
 //              holding both factor paths pays the registers of both (strided 512-point FP32
 //              kinetic pass: 254 VGPRs with both, 166 with the complex path alone)
+//   KIND_KIB : both factor paths, chosen at run time: the real-factor FP32 strided 512-point
+//              pass, which the compiler builds at 256 VGPRs + 50 spilled to AGPRs (1 wave per
+//              SIMD, 1.00 ms at 512^3) as KIND_KIR and at 254 VGPRs (2 waves, 0.72 ms) here
 //   KIND_POT : inverse, mid * exp(c (cons + V)) with complex V, forward
 //   KIND_GPE : inverse, mid * exp(c (cons + vext + g rho)), forward
 //   KIND_GPR : KIND_GPE with real c (imaginary time) only: without the complex-phase code the
@@ -45,7 +48,7 @@
 
 namespace lg {
 
-enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3, KIND_PRE = 4, KIND_GPR = 5, KIND_KIR = 6 };
+enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3, KIND_PRE = 4, KIND_GPR = 5, KIND_KIR = 6, KIND_KIB = 7 };
 
 
 // Threads per pencil (TPP) and values per thread (R) for each supported length L = R * TPP:
@@ -315,7 +318,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   __shared__ double red_lds[NT / 64][HGRID_RED_NCHAN];
   // kinetic pass on a strided axis: the axis's factor table staged in LDS (the transform's first
   // workgroup barrier publishes it)
-  constexpr bool KT_LDS = (KIND == KIND_KIN || KIND == KIND_KIR) && !CONTIG;
+  constexpr bool KT_LDS = (KIND == KIND_KIN || KIND == KIND_KIR || KIND == KIND_KIB) && !CONTIG;
   __shared__ __align__(16) double2 kt_lds[KT_LDS ? L : 1];
 
   const int tid = threadIdx.x;
@@ -441,7 +444,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
     if ((fl & PRE_ANY) && (fl & HGRID_PF_PRE_AFTER)) pre_ops();
   }
 
-  if constexpr (KIND == KIND_FFT || KIND == KIND_KIN || KIND == KIND_KIR) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_KIN || KIND == KIND_KIR || KIND == KIND_KIB) {
     if (fl & HGRID_PF_MID_KRED) {
       // |k|^2 of the pass axis formed in registers, bit for bit the table entry: a table load
       // per element, issued among the 32 live values of a 512-point pencil, came out serialised
@@ -451,13 +454,13 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
       for (int j = 0; j < R; ++j) acc[HGRID_RED_KIN] += (kk + k2_of<L>(t + TPP * j, a.k2_dk)) * cnorm2(v[j]);
     }
   }
-  if constexpr (KIND == KIND_KIN || KIND == KIND_KIR) {
+  if constexpr (KIND == KIND_KIN || KIND == KIND_KIR || KIND == KIND_KIB) {
     // FP64 tables whatever the storage precision; the factor is formed in FP64.  Real
     // factors (imaginary time) multiply in FP64 and only the product is rounded, so FP32
     // storage sees no fixed per-element error in K; complex ones are rounded once to C.
     const double2 *ka = KT_LDS ? (const double2 *)kt_lds : (const double2 *)a.ktab_a;
     const double2 f = cmul(((const double2 *)a.ktab_1)[c1], ((const double2 *)a.ktab_2)[c2]);
-    if constexpr (KIND == KIND_KIR) {
+    auto real_factors = [&]() {
       if constexpr (std::is_same<C, float2>::value) {
         // FP32 data: groups of four behind scheduling fences keep few FP64 factors live
 #pragma unroll
@@ -471,7 +474,8 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = cscale_d(v[j], ka[t + TPP * j].x * f.x);
       }
-    } else {
+    };
+    auto complex_factors = [&]() {
       // complex factors (real time): formed in FP64 in groups of four behind scheduling fences,
       // so that few FP64 table entries and products are live at once
 #pragma unroll
@@ -481,6 +485,14 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
         for (int j = j0; j < (j0 + 4 < R ? j0 + 4 : R); ++j) v[j] = cmul(v[j], round_to<C>(cmul(ka[t + TPP * j], f)));
       }
       __builtin_amdgcn_sched_barrier(0);
+    };
+    if constexpr (KIND == KIND_KIR) {
+      real_factors();
+    } else if constexpr (KIND == KIND_KIN) {
+      complex_factors();
+    } else {
+      if (fl & HGRID_PF_KIN_REAL) real_factors();
+      else complex_factors();
     }
     if (fl & HGRID_PF_KCUT) {
       // CFFT: components with |k - k0|^2 >= kmax^2 become exact zeros
@@ -619,6 +631,11 @@ template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long 
     case KIND_FFT: return contig ? launch_len<C, true, KIND_FFT, S>(p, L) : launch_len<C, false, KIND_FFT, S>(p, L);
     case KIND_KIN: return contig ? launch_len<C, true, KIND_KIN, S>(p, L) : launch_len<C, false, KIND_KIN, S>(p, L);
     case KIND_KIR: return contig ? launch_len<C, true, KIND_KIR, S>(p, L) : launch_len<C, false, KIND_KIR, S>(p, L);
+    case KIND_KIB:
+      if constexpr (std::is_same<C, float2>::value) {
+        if (!contig && L == 512) return launch<C, 512, false, KIND_KIB, S>(p);
+      }
+      return -1;
     case KIND_POT: return contig ? launch_len<C, true, KIND_POT, S>(p, L) : -1;
     case KIND_GPE: return contig ? launch_len<C, true, KIND_GPE, S>(p, L) : -1;
     case KIND_GPR: return contig ? launch_len<C, true, KIND_GPR, S>(p, L) : -1;
