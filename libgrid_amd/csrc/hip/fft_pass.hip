@@ -33,8 +33,8 @@ extern "C" int hgrid_fft_pass(const hgrid_pass *p) {
   int kind;
   if (p->flags & HGRID_PF_MID_KIN) {
     kind = (p->flags & HGRID_PF_KIN_REAL) ? lg::KIND_KIR : lg::KIND_KIN;
-    // real factors on a strided FP32 512-point axis: the kernel with both factor paths
-    if (kind == lg::KIND_KIR && p->prec == HGRID_C64 && p->axis != 2 && L == 512) kind = lg::KIND_KIB;
+    // FP32 real factors at the lengths where the kernel with both factor paths runs better
+    if (kind == lg::KIND_KIR && p->prec == HGRID_C64 && lg::kib_length(L)) kind = lg::KIND_KIB;
   }
   else if (p->flags & HGRID_PF_MID_POT) kind = lg::KIND_POT;
   else if (p->flags & HGRID_PF_MID_GPE) kind = p->c_im == 0.0 ? lg::KIND_GPR : lg::KIND_GPE;

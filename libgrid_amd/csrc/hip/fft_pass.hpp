@@ -24,9 +24,10 @@
 //   KIND_KIR : KIND_KIN with real factors (imaginary time).  Separate kinds because a kernel
 //              holding both factor paths pays the registers of both (strided 512-point FP32
 //              kinetic pass: 254 VGPRs with both, 166 with the complex path alone)
-//   KIND_KIB : both factor paths, chosen at run time: the real-factor FP32 strided 512-point
-//              pass, which the compiler builds at 256 VGPRs + 50 spilled to AGPRs (1 wave per
-//              SIMD, 1.00 ms at 512^3) as KIND_KIR and at 254 VGPRs (2 waves, 0.72 ms) here
+//   KIND_KIB : both factor paths, chosen at run time, for the FP32 real-factor passes the
+//              compiler builds worse as KIND_KIR (kib_length): the strided 512-point one came
+//              out at 256 VGPRs + 50 spilled to AGPRs (1 wave per SIMD, 1.00 ms at 512^3) as
+//              KIND_KIR and at 254 VGPRs (2 waves, 0.72 ms) here
 //   KIND_POT : inverse, mid * exp(c (cons + V)) with complex V, forward
 //   KIND_GPE : inverse, mid * exp(c (cons + vext + g rho)), forward
 //   KIND_GPR : KIND_GPE with real c (imaginary time) only: without the complex-phase code the
@@ -625,6 +626,23 @@ template <typename C, bool CONTIG, int KIND, typename S> int launch_len(const hg
 #undef LG_CASE
 }
 
+// the lengths whose FP32 real-factor kinetic pass runs with both factor paths (KIND_KIB): there
+// the real-only kernel comes out at fewer waves per SIMD (compiler resource report: 6, 8, 16
+// and 24 points one wave fewer, 512 and 1024 one wave instead of two)
+constexpr bool kib_length(long L) { return L == 6 || L == 8 || L == 16 || L == 24 || L == 512 || L == 1024; }
+
+template <typename C, bool CONTIG, typename S> int launch_kib(const hgrid_pass *p, long L) {
+  switch (L) {
+    case 6: return launch<C, 6, CONTIG, KIND_KIB, S>(p);
+    case 8: return launch<C, 8, CONTIG, KIND_KIB, S>(p);
+    case 16: return launch<C, 16, CONTIG, KIND_KIB, S>(p);
+    case 24: return launch<C, 24, CONTIG, KIND_KIB, S>(p);
+    case 512: return launch<C, 512, CONTIG, KIND_KIB, S>(p);
+    case 1024: return launch<C, 1024, CONTIG, KIND_KIB, S>(p);
+    default: return -1;
+  }
+}
+
 template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long L, int kind) {
   const bool contig = p->axis == 2;
   switch (kind) {
@@ -633,7 +651,7 @@ template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long 
     case KIND_KIR: return contig ? launch_len<C, true, KIND_KIR, S>(p, L) : launch_len<C, false, KIND_KIR, S>(p, L);
     case KIND_KIB:
       if constexpr (std::is_same<C, float2>::value) {
-        if (!contig && L == 512) return launch<C, 512, false, KIND_KIB, S>(p);
+        return contig ? launch_kib<C, true, S>(p, L) : launch_kib<C, false, S>(p, L);
       }
       return -1;
     case KIND_POT: return contig ? launch_len<C, true, KIND_POT, S>(p, L) : -1;

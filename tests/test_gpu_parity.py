@@ -210,6 +210,25 @@ def test_kinetic_phase(lg, orc, shape, k0, time):
     lg.grid_wf_free(w)
 
 
+@pytest.mark.parametrize("shape", [(512, 2, 6), (16, 24, 8), (1024, 2, 2), (24, 4, 4), (6, 4, 4), (8, 4, 4),
+                                   (256, 2, 4)])
+@pytest.mark.parametrize("time", [-0.05j, 0.02])
+def test_kinetic_phase_fp32(lg, orc, shape, time):
+    """complex64 storage, real (imaginary time) and complex kinetic factors on the outer axis, at
+    the lengths whose FP32 real-factor pass runs the kernel with both factor paths (6, 8, 16, 24,
+    512, 1024) and at one that does not (256), against the FP64 oracle of the rounded input."""
+    case(f"fp32 kinetic {shape} t={time}")
+    step, mass = 0.3, 1.3
+    a = rand_c(shape, 8).astype(np.complex64).astype(np.complex128)
+    w = new_wf(lg, a, step, mass)
+    lg.cgrid_set_precision(w.contents.grid, b"\x01")
+    lg.grid_wf_propagate_kinetic_fft(w, LG.c_complex.of(time))
+    err = rel_l2(LG.get_cgrid(w.contents.grid), orc.propagate_kinetic_fft(a, step, mass, (0, 0, 0), time))
+    print(f"rel L2 {err:.2e}")
+    assert err < 2e-7  # measured 2.5e-8 to 2.8e-8: the FP32 roundings of two transforms
+    lg.grid_wf_free(w)
+
+
 @pytest.mark.parametrize("tstep,cons", [(0.05, 0.0), (0.02 - 0.03j, 0.4), (-0.05j, -0.1)])
 def test_potential_phase(lg, orc, tstep, cons):
     case(f"potential t={tstep} cons={cons}")
