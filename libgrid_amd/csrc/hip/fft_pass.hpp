@@ -317,8 +317,9 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   constexpr unsigned CS = sizeof(S);  // bytes per stored element
   __shared__ __align__(16) Rl lds[LDS_N];
   __shared__ double red_lds[NT / 64][HGRID_RED_NCHAN];
-  // kinetic pass on a strided axis: the axis's factor table staged in LDS (the transform's first
-  // workgroup barrier publishes it)
+  // kinetic pass on a strided axis: the axis's factor table staged in LDS, published by the
+  // transform's first workgroup barrier, or by one of its own where the transform has none
+  // (one thread per pencil)
   constexpr bool KT_LDS = (KIND == KIND_KIN || KIND == KIND_KIR || KIND == KIND_KIB) && !CONTIG;
   __shared__ __align__(16) double2 kt_lds[KT_LDS ? L : 1];
 
@@ -388,6 +389,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   C v[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) v[j] = active ? widen<C>(bload<S>(rin, e0 * CS, (unsigned)j * estep * CS)) : mk<C>(0, 0);
+  if constexpr (KT_LDS && TPP == 1) __syncthreads();
 
   double acc[HGRID_RED_NCHAN] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
