@@ -170,11 +170,11 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
     """Wall time to |E_n - E_{n-1}| < 1e-10 Eh from the workload's initial state:
     (a) FP64 throughout; (b) FP32 storage until the energy change falls below 1e-7 Eh (the
     FP32 noise floor is ~1e-8), then FP64 to 1e-10; (c) as (b) with vector extrapolation every
-    64 steps in the FP64 stage; (d) as (b) with it in the FP32 stage.  Energies of all runs are
-    reported."""
+    64 steps in the FP64 stage; (d) as (b) with it in the FP32 stage; (e) with it in both.
+    Energies of all runs are reported."""
     step, g, dt, tol = prm["step"], prm["g"], prm["dt"], prm["tol"]
     out = {}
-    for mode in ("fp64", "fp32_then_fp64", "fp32_then_fp64e", "fp32e_then_fp64"):
+    for mode in ("fp64", "fp32_then_fp64", "fp32_then_fp64e", "fp32e_then_fp64", "fp32e_then_fp64e"):
         w = lib.grid_wf_alloc(n, n, n, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"gs")
         vr = lib.rgrid_alloc(n, n, n, step, LG.boundary("rgrid_value_outside_periodic"), None, b"vext")
         LG.set_cgrid(w.contents.grid, psi)
@@ -190,12 +190,12 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
             steps = lib.grid_wf_gpe_relax(w, vr, g, dt, tol, max_steps, C.byref(e))
             s32 = 0
         else:
-            if mode == "fp32e_then_fp64":
+            if mode in ("fp32e_then_fp64", "fp32e_then_fp64e"):
                 s32 = lib.grid_wf_gpe_relax_extrapolated(w, vr, g, dt, 1e-7, max_steps, C.byref(e), 64, None)
             else:
                 s32 = lib.grid_wf_gpe_relax(w, vr, g, dt, 1e-7, max_steps, C.byref(e))
             lib.cgrid_set_precision(w.contents.grid, b"\x00")
-            if mode == "fp32_then_fp64e":
+            if mode in ("fp32_then_fp64e", "fp32e_then_fp64e"):
                 steps = lib.grid_wf_gpe_relax_extrapolated(w, vr, g, dt, tol, max_steps, C.byref(e), 64, None)
             else:
                 steps = lib.grid_wf_gpe_relax(w, vr, g, dt, tol, max_steps, C.byref(e))
@@ -209,6 +209,7 @@ def ground_state(lib, LG, psi, vext, prm, n, max_steps):
     out["speedup_fp32_then_fp64"] = out["fp64"]["seconds"] / out["fp32_then_fp64"]["seconds"]
     out["speedup_fp32_then_fp64e"] = out["fp64"]["seconds"] / out["fp32_then_fp64e"]["seconds"]
     out["speedup_fp32e_then_fp64"] = out["fp64"]["seconds"] / out["fp32e_then_fp64"]["seconds"]
+    out["speedup_fp32e_then_fp64e"] = out["fp64"]["seconds"] / out["fp32e_then_fp64e"]["seconds"]
     out["tol_Eh"] = tol
     return out
 
