@@ -44,10 +44,13 @@ extern "C" int hgrid_fft_pass(const hgrid_pass *p) {
   constexpr unsigned PRE_KEEP = HGRID_PF_PRESCALE | HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF;
   if (kind == lg::KIND_FFT && p->axis == 2 && (p->flags & PRE_ONLY) == PRE_ONLY && !(p->flags & ~(PRE_ONLY | PRE_KEEP)))
     kind = lg::KIND_PRE;
+  // a plain transform (pending scale, post factor and output norm at most)
+  if (kind == lg::KIND_FFT && !(p->flags & ~(HGRID_PF_PRESCALE | HGRID_PF_POST_RED))) kind = lg::KIND_FY;
   const unsigned skip = p->flags & (HGRID_PF_POT_NOFFT1 | HGRID_PF_POT_NOFFT2);
   if (skip && (kind != lg::KIND_POT || skip == (HGRID_PF_POT_NOFFT1 | HGRID_PF_POT_NOFFT2))) return -1;
-  if ((kind == lg::KIND_FFT || kind == lg::KIND_PRE) && p->nfft != 1) return -1;
-  if (kind != lg::KIND_FFT && kind != lg::KIND_PRE && p->nfft != (skip ? 1 : 2)) return -1;
+  const bool one = kind == lg::KIND_FFT || kind == lg::KIND_PRE || kind == lg::KIND_FY;
+  if (one && p->nfft != 1) return -1;
+  if (!one && p->nfft != (skip ? 1 : 2)) return -1;
   if (p->prec == HGRID_C128) return lg::launch_f64(p, L, kind);
   if (p->prec == HGRID_C64) return lg::launch_f32(p, L, kind);
   if (p->prec == HGRID_C64X) return lg::launch_f32s_f64(p, L, kind);

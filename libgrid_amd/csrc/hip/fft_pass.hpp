@@ -34,6 +34,9 @@
 //              256-point FP64 pass fits 3 waves per SIMD (166 instead of 224 VGPRs)
 //   KIND_PRE : one transform on a contiguous axis, then |psi|^2 / V_eff / reductions of the
 //              result, nothing stored (the start-of-step pass over a state kept transformed)
+//   KIND_FY  : KIND_FFT with at most the pending scale, the post factor and the output norm
+//              (the y passes of a step, plain transforms): without the input-side paths the
+//              kernel keeps fewer registers (strided 512-point FP32: 210 -> 112 VGPRs)
 // Inverse transforms use conj(DFT(conj(x))), so one set of butterflies serves both signs.
 // Reference operators this replaces: cgrid_fft / cgrid_inverse_fft (src/cgrid.c:1822-1846,
 // src/cgrid_fft.c:101-146), the kinetic phase (src/grid_wf_fft.c:141-193), the potential
@@ -49,7 +52,8 @@
 
 namespace lg {
 
-enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3, KIND_PRE = 4, KIND_GPR = 5, KIND_KIR = 6, KIND_KIB = 7 };
+enum { KIND_FFT = 0, KIND_KIN = 1, KIND_POT = 2, KIND_GPE = 3, KIND_PRE = 4, KIND_GPR = 5, KIND_KIR = 6, KIND_KIB = 7,
+       KIND_FY = 8 };
 
 
 // Threads per pencil (TPP) and values per thread (R) for each supported length L = R * TPP:
@@ -378,7 +382,9 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   const unsigned estep = (unsigned)TPP * astride;              // flat index step per j
   // KIND_PRE: a KIND_FFT pass whose other flags are known to be clear, compiled without them
   constexpr unsigned PRE_KEEP = HGRID_PF_PRESCALE | HGRID_PF_PRE_RHO | HGRID_PF_PRE_RED | HGRID_PF_PRE_VEFF;
-  const unsigned fl = KIND == KIND_PRE ? (a.flags & PRE_KEEP) | HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE : a.flags;
+  const unsigned fl = KIND == KIND_PRE  ? (a.flags & PRE_KEEP) | HGRID_PF_PRE_AFTER | HGRID_PF_NOSTORE
+                      : KIND == KIND_FY ? a.flags & (HGRID_PF_PRESCALE | HGRID_PF_POST_RED)
+                                        : a.flags;
   const C *__restrict__ tw = (const C *)a.tw;
   const auto rin = make_rsrc(a.in, (unsigned long)ntot * CS);
   if constexpr (KT_LDS) {
@@ -393,7 +399,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
 
   double acc[HGRID_RED_NCHAN] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
-  if constexpr (KIND == KIND_FFT || KIND == KIND_POT || KIND == KIND_PRE) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_POT || KIND == KIND_PRE || KIND == KIND_FY) {
     if (fl & HGRID_PF_PRESCALE) {
       const C s = mk<C>((Rl)a.scale[0], (Rl)a.scale[1]);
 #pragma unroll
@@ -552,18 +558,18 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
 
   if constexpr (KIND == KIND_POT) {
     if (!(fl & HGRID_PF_POT_NOFFT2)) fft_core<C, L, CONTIG, WIDE>(v, t, q, a.dir2 > 0, tw, lds);
-  } else if constexpr (KIND != KIND_FFT && KIND != KIND_PRE) {
+  } else if constexpr (KIND != KIND_FFT && KIND != KIND_PRE && KIND != KIND_FY) {
     fft_core<C, L, CONTIG, WIDE>(v, t, q, a.dir2 > 0, tw, lds);
   }
 
-  if constexpr (KIND == KIND_FFT) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_FY) {
     if (a.post != 1.0) {
       const Rl post = (Rl)a.post;
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = cscale(v[j], post);
     }
   }
-  if constexpr (KIND == KIND_FFT || KIND == KIND_GPE || KIND == KIND_GPR) {
+  if constexpr (KIND == KIND_FFT || KIND == KIND_GPE || KIND == KIND_GPR || KIND == KIND_FY) {
     // the stored values (a merged relaxation step renormalises from the output of its potential
     // pass; kinetic and complex-potential passes never reduce their output, and the accumulation
     // would cost the strided 512-point kinetic pass its second wave per SIMD)
@@ -649,6 +655,7 @@ template <typename C, typename S = C> int launch_prec(const hgrid_pass *p, long 
   const bool contig = p->axis == 2;
   switch (kind) {
     case KIND_FFT: return contig ? launch_len<C, true, KIND_FFT, S>(p, L) : launch_len<C, false, KIND_FFT, S>(p, L);
+    case KIND_FY: return contig ? launch_len<C, true, KIND_FY, S>(p, L) : launch_len<C, false, KIND_FY, S>(p, L);
     case KIND_KIN: return contig ? launch_len<C, true, KIND_KIN, S>(p, L) : launch_len<C, false, KIND_KIN, S>(p, L);
     case KIND_KIR: return contig ? launch_len<C, true, KIND_KIR, S>(p, L) : launch_len<C, false, KIND_KIR, S>(p, L);
     case KIND_KIB:
