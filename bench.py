@@ -147,9 +147,13 @@ def fp64_512_bench(lib, LG, steps=10, warmup=3):
     return out
 
 
-# template instantiation of each pass kind in the rocprofv3 summaries (precision, length, layout, kind)
-PASS_SYMBOL = {name: f"256, {layout}, {kind}, HIP_vector_type<double, 2u> >"
-               for name, layout, kind in (("y_fft", "false", 0), ("x_kin", "false", 1), ("z_gpe", "true", 3))}
+# template instantiation of each pass of the 256^3 FP64 imaginary-time step in the rocprofv3
+# summaries (length, contiguous, kernel kind, storage type, wide tiles): the kinds of
+# libgrid_amd/csrc/hip/fft_pass.hpp are 8 plain transform (y), 6 real kinetic factors (x),
+# 5 imaginary-time GPE (z), 4 start-of-step (z)
+PASS_SYMBOL = {name: f"256, {layout}, {kind}, HIP_vector_type<double, 2u>, false>"
+               for name, layout, kind in (("y_fft", "false", 8), ("x_kin", "false", 6), ("z_gpe", "true", 5),
+                                          ("z_inv_pre", "true", 4))}
 
 
 def profiled_traffic(pass_name, n):
