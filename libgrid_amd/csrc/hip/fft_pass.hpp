@@ -341,8 +341,8 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   const long npencils = ntot / L;
   // XCD-aware order on the y axis: workgroup b runs on XCD b % 8, so the first 8 (nb / 8)
   // workgroups are renumbered to give each XCD a contiguous run of tiles (the tail keeps its
-  // number).  At 512^3 FP64 the y pass went from 916 to 780 us; on the x axis the same order was
-  // slightly slower, so x keeps the hardware order.
+  // number).  At 512^3 FP64 the y pass went from 916 to 780 us; on the x axis and on the
+  // contiguous axis the same order was slower, so those keep the hardware order.
   long blk = blockIdx.x;
   if constexpr (!CONTIG) {
     if (a.axis == 1) {
