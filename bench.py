@@ -471,8 +471,8 @@ def main():
                          "one graph and instantiated another)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "the per-GPU kernels and the sharding design are described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU: the library drives one GPU per process and has no sharded "
+              f"path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
     import libgrid_amd as LG
