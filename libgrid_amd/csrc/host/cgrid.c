@@ -88,6 +88,7 @@ void cgrid_free(cgrid *grid) {
     if (d->dscal) lg_hip(hgrid_free(d->dscal), "hipFree");
     if (d->work) lg_hip(hgrid_free(d->work), "hipFree");
     if (d->rho) lg_hip(hgrid_free(d->rho), "hipFree");
+    if (d->alt) lg_hip(hgrid_free(d->alt), "hipFree");
     free_ktab(&d->kin);
     free_ktab(&d->kin2);
     free_ktab(&d->kin_energy);
@@ -144,6 +145,7 @@ static void upload(cgrid *g) {
   }
   d->valid |= GV_DEV;
   d->zpend = 0;
+  d->stamp++;
 }
 
 static void download(cgrid *g) {
@@ -180,6 +182,7 @@ void lg_c_materialize(cgrid *g) {
 void *lg_c_raw(cgrid *g, const double **scale) {
   grid_dev *d = lg_cdev(g);
   if (!(d->valid & GV_DEV)) upload(g);
+  d->stamp++; /* the fused step writes it */
   *scale = d->pending ? d->dscal + GS_SCALE : NULL;
   return d->d;
 }
@@ -195,12 +198,35 @@ void *lg_c_in(cgrid *g, const double **scale) {
 
 void *lg_c_in_flushed(cgrid *g) { return lg_c_in(g, NULL); }
 
-void *lg_c_inout(cgrid *g, const double **scale) { return lg_c_in(g, scale); }
+void *lg_c_inout(cgrid *g, const double **scale) {
+  void *p = lg_c_in(g, scale);
+  g->dev->stamp++;
+  return p;
+}
 
 void *lg_c_out(cgrid *g) {
   grid_dev *d = lg_cdev(g);
   d->zpend = 0; /* about to be overwritten */
+  d->stamp++;
   return d->d;
+}
+
+const void *lg_c_in_as(cgrid *g, int prec) {
+  const void *src = lg_c_in_flushed(g);
+  grid_dev *d = g->dev;
+  /* FP32 storage with either arithmetic: the same bytes */
+  if (lg_prec_size(prec) == lg_prec_size(d->prec)) return src;
+  if (d->alt && d->alt_prec == prec && d->alt_stamp == d->stamp) return d->alt;
+  if (d->alt && lg_prec_size(d->alt_prec) != lg_prec_size(prec)) {
+    lg_hip(hgrid_free(d->alt), "hipFree");
+    d->alt = NULL;
+  }
+  if (!d->alt) lg_hip(hgrid_malloc(&d->alt, d->n * lg_prec_size(prec)), "hipMalloc (converted grid)");
+  if (prec == HGRID_C128) lg_hip(hgrid_c64_to_c128(d->alt, src, d->n), "convert");
+  else lg_hip(hgrid_c128_to_c64(d->alt, src, d->n), "convert");
+  d->alt_prec = prec;
+  d->alt_stamp = d->stamp;
+  return d->alt;
 }
 
 /* After a kernel wrote the device array: the host copy is stale.  scale_consumed = the kernel
@@ -208,6 +234,7 @@ void *lg_c_out(cgrid *g) {
 void lg_c_written(cgrid *g, int scale_consumed) {
   grid_dev *d = g->dev;
   d->valid = GV_DEV;
+  d->stamp++;
   d->zpend = 0;
   if (scale_consumed && d->pending) {
     lg_hip(hgrid_scalar_set(d->dscal + GS_SCALE, 1.0, 0.0), "scalar_set");
@@ -221,6 +248,7 @@ void lg_c_set_pending(cgrid *g, double re, double im) {
   lg_hip(hgrid_scalar_mul(d->dscal + GS_SCALE, re, im), "scalar_mul");
   d->pending = 1;
   d->valid = GV_DEV;
+  d->stamp++;
 }
 
 void lg_c_flush(cgrid *g) {
@@ -245,6 +273,7 @@ void cgrid_host_sync(cgrid *grid) { lg_c_host_sync(grid); }
 void cgrid_host_modified(cgrid *grid) {
   grid_dev *d = grid->dev;
   d->valid = GV_HOST;
+  d->stamp++;
   d->zpend = 0;
   if (d->pending) {
     lg_hip(hgrid_scalar_set(d->dscal + GS_SCALE, 1.0, 0.0), "scalar_set");
@@ -256,6 +285,7 @@ void cgrid_set_precision(cgrid *grid, char prec) {
   int p = prec == GRID_PREC_SINGLE ? HGRID_C64 : prec == GRID_PREC_SINGLE_STORAGE ? HGRID_C64X : HGRID_C128;
   grid_dev *d = grid->dev;
   if (d->prec == p) return;
+  d->stamp++;
   if (lg_prec_size(p) == lg_prec_size(d->prec)) {
     /* FP32 storage either way: only the arithmetic of the passes changes */
     d->prec = p;

@@ -152,8 +152,8 @@ void grid_wf_propagate_potential(wf *gwf, REAL complex tstep, cgrid *potential, 
    * dependent time of the CN propagators is not built */
   if (gwf->ts_func && gwf->propagator >= WF_2ND_ORDER_CN)
     lg_fatal("grid_wf_propagate_potential: absorbing boundaries with the CN propagators are not supported by this build");
-  if (potential->dev->prec != g->dev->prec) cgrid_set_precision(potential, cgrid_get_precision(g));
-  const void *pot = lg_c_in_flushed(potential);
+  /* the potential keeps its own precision; a copy in the state's storage precision is used */
+  const void *pot = lg_c_in_as(potential, g->dev->prec);
   const double *scale;
   void *psi = lg_c_inout(g, &scale);
   /* c = -i tstep / hbar */
@@ -513,6 +513,7 @@ static void fused_step(wf *gwf, const step_args *sa) {
   }
   if (fin.njobs || fin.scale) lg_hip(hgrid_step_finalize(&fin), "step_finalize");
   d->valid = GV_DEV;
+  d->stamp++;
   d->pending = sa->renorm_target > 0.0;
   d->zpend = defer;
   d->kpend = stage == 1 || stage == 2;
@@ -612,8 +613,9 @@ static void propagate_split(wf *gwf, cgrid *potential, REAL complex time, int re
   if (potential) {
     if (potential->nx != g->nx || potential->ny != g->ny || potential->nz != g->nz)
       lg_fatal("grid_wf_propagate: potential and wavefunction grids differ in size");
-    if (potential->dev->prec != g->dev->prec) cgrid_set_precision(potential, cgrid_get_precision(g));
-    sa.pot = lg_c_in_flushed(potential);
+    /* the potential keeps its own precision (a copy in the state's storage precision is used),
+     * so an FP64 state propagated with it later still sees the FP64 values */
+    sa.pot = lg_c_in_as(potential, g->dev->prec);
   }
   /* 4th order (reference src/grid_wf.c:433-457): the outer V(t/6) stages ride on the first
    * and last z passes of a fused K(t/2) V(2t/3) K(t/2).  The reference also forms

@@ -49,6 +49,10 @@ typedef struct grid_dev {
   int kpend;               /* inside a merged relaxation batch: the array holds the state before
                               the last half kinetic factor of the step (never seen by callers) */
   ktab_cache kin_energy;   /* |k|^2 tables for energy reductions */
+  unsigned long stamp;     /* bumped whenever the device array may have been written */
+  void *alt;               /* the data converted to another storage precision (lg_c_in_as) */
+  int alt_prec;
+  unsigned long alt_stamp; /* stamp of the data alt was converted from */
   double last_energy_parts[4];
   int have_gpe_energy;
 } grid_dev;
@@ -66,6 +70,9 @@ size_t lg_prec_size(int prec);
 grid_dev *lg_cdev(cgrid *g);                    /* allocate device state if needed */
 void *lg_c_in(cgrid *g, const double **scale);  /* device data for reading; scale = pending or NULL */
 void *lg_c_in_flushed(cgrid *g);                /* device data with any pending scale applied */
+/* device data with any pending scale applied, in storage precision prec: the grid's own array,
+ * or a converted copy kept beside it (the grid keeps its own precision) */
+const void *lg_c_in_as(cgrid *g, int prec);
 void *lg_c_out(cgrid *g);                       /* device data to be overwritten (no upload) */
 void *lg_c_inout(cgrid *g, const double **scale);
 void lg_c_written(cgrid *g, int scale_consumed);

@@ -373,6 +373,41 @@ def test_gpe_step_matches_composition(lg, orc, normalize):
     lg.rgrid_free(vr)
 
 
+def test_potential_keeps_its_precision(lg, orc):
+    """A complex potential used with an FP32 state keeps its FP64 values: the state's storage
+    precision gets a converted copy.  Propagating an FP64 state with the same potential afterwards
+    matches the oracle at the FP64 bar (had the potential been rounded to FP32 in place, the
+    result would be off by ~5e-10)."""
+    case("potential precision after FP32 use")
+    shape, step, mass, dt = (16, 24, 32), 0.3, 1.0, 0.02
+    rng = np.random.default_rng(21)
+    psi = rand_c(shape, 22) * 0.1 + 1.0
+    pot = (rng.uniform(0, 1, shape) + 0.05j * rng.uniform(0, 1, shape)).astype(np.complex128)
+    vp = new_cgrid(lg, pot, step)
+    w32 = new_wf(lg, psi, step, mass)
+    lg.cgrid_set_precision(w32.contents.grid, b"\x01")
+    for _ in range(2):
+        lg.grid_wf_propagate(w32, vp, LG.c_complex.of(dt))
+    assert lg.cgrid_get_precision(vp) == b"\x00"
+    assert rel_l2(LG.get_cgrid(vp), pot) == 0.0
+    w64 = new_wf(lg, psi, step, mass)
+    ref = psi.copy()
+    for _ in range(3):
+        lg.grid_wf_propagate(w64, vp, LG.c_complex.of(dt))
+        orc.wf_propagate_inplace(ref, pot, step, mass, (0, 0, 0), dt)
+    assert rel_l2(LG.get_cgrid(w64.contents.grid), ref) < TOL
+    # the potential written after the FP32 use is seen by the next FP32 step (the copy follows it)
+    pot2 = pot * 0.5
+    LG.set_cgrid(vp, pot2)
+    ref32 = LG.get_cgrid(w32.contents.grid).copy()
+    lg.grid_wf_propagate(w32, vp, LG.c_complex.of(dt))
+    orc.wf_propagate_inplace(ref32, pot2, step, mass, (0, 0, 0), dt)
+    assert rel_l2(LG.get_cgrid(w32.contents.grid), ref32) < 2e-6
+    for w in (w32, w64):
+        lg.grid_wf_free(w)
+    lg.cgrid_free(vp)
+
+
 def test_single_precision_path(lg, orc):
     """complex64 storage with FP64 twiddles / phase arguments / reductions: error vs FP64 oracle."""
     psi0, v, prm = W.wavepacket_harmonic(seed=1, n=64, step=0.4)

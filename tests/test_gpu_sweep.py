@@ -1,6 +1,8 @@
-"""Parity sweep of the fused GPE step over every supported axis length on each axis, in the three
+"""Parity sweeps over every supported axis length on each axis.  The fused GPE step in the three
 storage precisions, for imaginary and real time: three steps of grid_wf_gpe_propagate (with
 renormalisation) against the oracle's composition of reference calls from the same FP64 input.
+The propagators with a complex potential (2nd order in the three precisions, CFFT and 4th
+order in FP64), the same potential grid serving every precision in turn.
 It reaches every pass kernel kind and geometry the library picks from the shape: one-thread,
 two-stage and three-stage pencils, wide outer-axis tiles, partial workgroups, radix-3 lengths and
 squeezed 2-D shapes.
@@ -50,3 +52,47 @@ def test_gpe_steps_every_length(lg, orc, shape):
             lg.grid_wf_free(w)
             lg.rgrid_free(vr)
             assert err < bar, f"{shape} precision {prec[0]} t={time}: rel L2 {err:.3e}"
+
+
+def kmax_for(shape, step, frac):
+    """frac times the largest |k| on the grid (Nyquist along every non-trivial axis)."""
+    return frac * np.sqrt(sum((np.pi / step) ** 2 for n in shape if n > 1))
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_propagators_every_length(lg, orc, shape):
+    """grid_wf_propagate with a complex potential over the same shapes: the 2nd-order FFT step in
+    the three precisions (complex-potential and complex kinetic kernels), its k-space cutoff
+    (CFFT, kmax = 0.6 of the corner |k|) and the 4th-order step, both in FP64."""
+    print(f"[case] propagators {shape}", flush=True)
+    sys.stdout.flush()
+    rng = np.random.default_rng(7 + sum(shape))
+    step, mass, time = 0.3, 1.2, 0.02
+    psi = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.1 + 1.0
+    pot = (rng.uniform(0, 1, shape) + 0.05j * rng.uniform(0, 1, shape)).astype(np.complex128)
+    kmax = kmax_for(shape, step, 0.6)
+    cases = [(LG.WF_2ND_ORDER_FFT, p, bar, None) for p, bar in BARS.items()]
+    cases += [(LG.WF_2ND_ORDER_CFFT, b"\x00", 1e-12, kmax), (LG.WF_4TH_ORDER_FFT, b"\x00", 1e-12, None)]
+    vp = lg.cgrid_alloc(*shape, step, LG.boundary("cgrid_value_outside_periodic"), None, b"pot")
+    LG.set_cgrid(vp, pot)
+    for prop, prec, bar, km in cases:
+        ref = psi.copy()
+        for _ in range(3):
+            if prop == LG.WF_2ND_ORDER_FFT:
+                orc.wf_propagate_inplace(ref, pot, step, mass, (0, 0, 0), time)
+            elif prop == LG.WF_2ND_ORDER_CFFT:
+                orc.wf_propagate_cfft_inplace(ref, pot, step, mass, (0, 0, 0), time, km)
+            else:
+                orc.wf_propagate_4th_inplace(ref, pot, step, mass, (0, 0, 0), time)
+        w = lg.grid_wf_alloc(*shape, step, mass, LG.WF_PERIODIC_BOUNDARY, prop, b"w")
+        if km is not None:
+            w.contents.kmax = km
+        LG.set_cgrid(w.contents.grid, psi)
+        if prec != b"\x00":
+            lg.cgrid_set_precision(w.contents.grid, prec)
+        for _ in range(3):
+            lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
+        err = rel_l2(LG.get_cgrid(w.contents.grid), ref)
+        lg.grid_wf_free(w)
+        assert err < bar, f"{shape} propagator {prop} precision {prec[0]}: rel L2 {err:.3e}"
+    lg.cgrid_free(vp)
