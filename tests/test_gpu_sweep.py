@@ -96,3 +96,32 @@ def test_propagators_every_length(lg, orc, shape):
         lg.grid_wf_free(w)
         assert err < bar, f"{shape} propagator {prop} precision {prec[0]}: rel L2 {err:.3e}"
     lg.cgrid_free(vp)
+
+
+FFT_BARS = {b"\x00": 1e-12, b"\x01": 1e-6, b"\x02": 2e-7}
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_fft_every_length(lg, orc, shape):
+    """cgrid_fft and cgrid_inverse_fft_norm over the same shapes in the three storage precisions,
+    against the oracle of the input the grid holds (rounded to complex64 for the FP32 storage
+    modes)."""
+    print(f"[case] fft {shape}", flush=True)
+    sys.stdout.flush()
+    rng = np.random.default_rng(3 + sum(shape))
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    step = 0.3
+    for prec, bar in FFT_BARS.items():
+        x = a if prec == b"\x00" else a.astype(np.complex64).astype(np.complex128)
+        g = lg.cgrid_alloc(*shape, step, LG.boundary("cgrid_value_outside_periodic"), None, b"g")
+        LG.set_cgrid(g, a)
+        if prec != b"\x00":
+            lg.cgrid_set_precision(g, prec)
+        lg.cgrid_fft(g)
+        fx = orc.fft3d(x, -1)
+        err_f = rel_l2(LG.get_cgrid(g), fx)
+        lg.cgrid_inverse_fft_norm(g)
+        err_i = rel_l2(LG.get_cgrid(g), x)
+        lg.cgrid_free(g)
+        assert err_f < bar, f"{shape} precision {prec[0]} forward: rel L2 {err_f:.3e}"
+        assert err_i < 2 * bar, f"{shape} precision {prec[0]} round trip: rel L2 {err_i:.3e}"
