@@ -125,3 +125,36 @@ def test_fft_every_length(lg, orc, shape):
         lg.cgrid_free(g)
         assert err_f < bar, f"{shape} precision {prec[0]} forward: rel L2 {err_f:.3e}"
         assert err_i < 2 * bar, f"{shape} precision {prec[0]} round trip: rel L2 {err_i:.3e}"
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_energies_every_length(lg, orc, shape):
+    """The reductions over the same shapes (FP64): norm, potential expectation, grid_wf_energy by
+    both methods (FFT kinetic energy, and the finite-difference default), the GP energy, and
+    the energy the fused GPE step records for its start state."""
+    print(f"[case] energies {shape}", flush=True)
+    sys.stdout.flush()
+    rng = np.random.default_rng(5 + sum(shape))
+    step, mass, g = 0.3, 1.0, 0.3
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape) + 2.0
+    v = rng.uniform(0, 1, shape)
+    w = lg.grid_wf_alloc(*shape, step, mass, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"w")
+    LG.set_cgrid(w.contents.grid, a)
+    vr = lg.rgrid_alloc(*shape, step, LG.boundary("rgrid_value_outside_periodic"), None, b"v")
+    LG.set_rgrid(vr, v)
+    checks = [
+        ("norm", lg.grid_wf_norm(w), orc.integral_of_square(a, step)),
+        ("expectation", lg.grid_grid_expectation_value(w.contents.grid, vr), orc.grid_expectation(a, v, step)),
+    ]
+    lg.grid_wf_analyze_method(b"\x01")
+    checks.append(("energy fft", lg.grid_wf_energy(w, vr), orc.energy_fft(a, v, step, mass)))
+    lg.grid_wf_analyze_method(b"\x00")
+    checks.append(("energy fd", lg.grid_wf_energy(w, vr),
+                   orc.kinetic_energy_fd_periodic(a, step, mass) + orc.grid_expectation(a, v, step)))
+    checks.append(("gpe energy", lg.grid_wf_gpe_energy(w, vr, g), orc.gpe_energy(a, v, g, step, mass)))
+    lg.grid_wf_gpe_propagate(w, vr, g, LG.c_complex.of(-0.01j), 1)
+    checks.append(("step energy", lg.grid_wf_gpe_last_energy(w), orc.gpe_energy(a, v, g, step, mass)))
+    lg.grid_wf_free(w)
+    lg.rgrid_free(vr)
+    for name, got, ref in checks:
+        assert abs(got - ref) <= 1e-12 * abs(ref), f"{shape} {name}: {got!r} against {ref!r}"
