@@ -158,3 +158,38 @@ def test_energies_every_length(lg, orc, shape):
     lg.rgrid_free(vr)
     for name, got, ref in checks:
         assert abs(got - ref) <= 1e-12 * abs(ref), f"{shape} {name}: {got!r} against {ref!r}"
+
+
+@pytest.mark.parametrize("shape", [s for s in SHAPES if min(s) > 1], ids=lambda s: "x".join(map(str, s)))
+def test_merged_relaxation_every_length(lg, orc, shape):
+    """Linear 3-D relaxation (g = 0) over the 3-D shapes: its batches merge the kinetic half-steps
+    of consecutive steps (kernels of the merged stages, full-step kinetic tables).  With tol = 0
+    it runs all 20 steps (a batch of 16 and one of 4); the state must be the oracle's 20 whole
+    reference steps, each renormalised, and the energy the oracle's."""
+    import ctypes as C
+
+    print(f"[case] merged relaxation {shape}", flush=True)
+    sys.stdout.flush()
+    step, dt = 0.35, 0.1
+    x = [(np.arange(n) - n // 2) * step for n in shape]
+    r2 = x[0][:, None, None] ** 2 + x[1][None, :, None] ** 2 + x[2][None, None, :] ** 2
+    v = 0.02 * r2 - 0.3 * np.exp(-r2 / 4.0)
+    rng = np.random.default_rng(41 + sum(shape))
+    psi = (np.exp(-r2 / 6.0) * (1 + 0.2 * rng.normal(size=shape)) + 0.05).astype(np.complex128)
+    w = lg.grid_wf_alloc(*shape, step, 1.0, LG.WF_PERIODIC_BOUNDARY, LG.WF_2ND_ORDER_FFT, b"m")
+    LG.set_cgrid(w.contents.grid, psi)
+    r = lg.rgrid_alloc(*shape, step, LG.boundary("rgrid_value_outside_periodic"), None, b"v")
+    LG.set_rgrid(r, v)
+    e = C.c_double()
+    steps = lg.grid_wf_gpe_relax(w, r, 0.0, dt, 0.0, 20, C.byref(e))
+    got = LG.get_cgrid(w.contents.grid)
+    lg.grid_wf_free(w)
+    lg.rgrid_free(r)
+    a = np.ascontiguousarray(psi.copy())
+    orc.gpe_step_inplace(a, v, 0.0, step, 1.0, (0, 0, 0), 0.0, normalize=True, target=1.0)  # normalise only
+    for _ in range(20):
+        orc.gpe_step_inplace(a, v, 0.0, step, 1.0, (0, 0, 0), -1j * dt, normalize=True, target=1.0)
+    eo = orc.gpe_energy(a, v, 0.0, step, 1.0)
+    assert steps == 20
+    assert rel_l2(got, a) < 1e-12
+    assert abs(e.value - eo) <= 1e-12 * abs(eo)
