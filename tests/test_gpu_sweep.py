@@ -193,3 +193,35 @@ def test_merged_relaxation_every_length(lg, orc, shape):
     assert steps == 20
     assert rel_l2(got, a) < 1e-12
     assert abs(e.value - eo) <= 1e-12 * abs(eo)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_absorbing_boundary_every_length(lg, orc, shape):
+    """grid_wf_boundary + grid_wf_propagate (2nd-order FFT and CFFT, FP64) over the same shapes:
+    two fused steps kept in real space and the merge sweep, against the oracle."""
+    print(f"[case] absorbing {shape}", flush=True)
+    sys.stdout.flush()
+    rng = np.random.default_rng(9 + sum(shape))
+    step, mass, time = 0.3, 1.0, 0.03 - 0.005j
+    psi = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.1 + 1.0
+    pot = (0.5 * rng.uniform(0, 1, shape) + 0.01j * rng.uniform(0, 1, shape)).astype(np.complex128)
+    bounds = []
+    for n in shape:
+        bounds += [0, 0] if n == 1 else [n // 4, n - 1 - n // 4]
+    vp = lg.cgrid_alloc(*shape, step, LG.boundary("cgrid_value_outside_periodic"), None, b"pot")
+    LG.set_cgrid(vp, pot)
+    for prop, km in ((LG.WF_2ND_ORDER_FFT, None), (LG.WF_2ND_ORDER_CFFT, kmax_for(shape, step, 0.6))):
+        w = lg.grid_wf_alloc(*shape, step, mass, LG.WF_PERIODIC_BOUNDARY, prop, b"w")
+        if km is not None:
+            w.contents.kmax = km
+        LG.set_cgrid(w.contents.grid, psi)
+        lg.grid_wf_boundary(w, None, *bounds)
+        ref = psi.copy()
+        for _ in range(3):
+            lg.grid_wf_propagate(w, vp, LG.c_complex.of(time))
+            orc.wf_propagate_absorb_inplace(ref, pot, step, mass, (0, 0, 0), time, tuple(bounds), km)
+        err = rel_l2(LG.get_cgrid(w.contents.grid), ref)
+        lg.grid_wf_boundary(w, None, 0, 0, 0, 0, 0, 0)
+        lg.grid_wf_free(w)
+        assert err < 1e-12, f"{shape} propagator {prop} bounds {bounds}: rel L2 {err:.3e}"
+    lg.cgrid_free(vp)
