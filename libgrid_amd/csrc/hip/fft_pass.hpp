@@ -285,14 +285,27 @@ template <> __device__ __forceinline__ double2 phase_factor<double2>(double cr, 
 __device__ __forceinline__ double real_decay(double x) { return exp_f64(x); }
 
 template <> __device__ __forceinline__ float2 phase_factor<float2>(double cr, double ci, double vr, double vi) {
-  // FP64 argument, reduced mod 2 pi in FP64, then FP32 sincos / exp
+  // FP64 argument, reduced by multiples of pi/2 in FP64, then FP32 sin / cos / exp
   const double ar = cr * vr - ci * vi, ai = cr * vi + ci * vr;
-  const double twopi = 6.283185307179586476925286766559;
-  const double red = ai - twopi * rint(ai * (1.0 / twopi));
   float s, c;
-  sincosf((float)red, &s, &c);
+  sincos_f32(ai, &s, &c);
   const float m = expf((float)ar);
   return make_float2(m * c, m * s);
+}
+
+// exp(i x) (FP64 argument): the factor exp(c V) of real time (c_re = 0) with a real V, whose
+// exponent c V has no real part.  Equal to phase_factor(0, c_im, V, 0) bit for bit, without
+// evaluating exp(0).
+template <typename C> __device__ __forceinline__ C unit_phase(double x);
+template <> __device__ __forceinline__ double2 unit_phase<double2>(double x) {
+  double s, c;
+  sincos_f64(x, &s, &c);
+  return make_double2(c, s);
+}
+template <> __device__ __forceinline__ float2 unit_phase<float2>(double x) {
+  float s, c;
+  sincos_f32(x, &s, &c);
+  return make_float2(c, s);
 }
 
 // C is the arithmetic type, S the storage type of psi (and of a complex potential): S = C, or
@@ -550,8 +563,13 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = cscale_d(v[j], (double)mid * real_decay(a.c_re * vr[j]));
       } else if constexpr (KIND != KIND_GPR) {
+        if (a.c_re == 0.0) {
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, vr[j], 0.0));
+          for (int j = 0; j < R; ++j) v[j] = cmul(cscale(v[j], mid), unit_phase<C>(a.c_im * vr[j]));
+        } else {
+#pragma unroll
+          for (int j = 0; j < R; ++j) v[j] = cmul(cscale(v[j], mid), phase_factor<C>(a.c_re, a.c_im, vr[j], 0.0));
+        }
       }
     }
   }

@@ -50,8 +50,11 @@ LG_HD double exp_f64(double x) {
 }
 
 LG_HD void sincos_f64(double x, double *s, double *c) {
-  if (!(x - x == 0.0)) {  // inf or nan
-    *s = *c = x - x;
+  // inf or nan, by a class test: the form x - x != 0 breaks when x is a product inlined from
+  // the caller, since the compiler may contract (a b) - (a b) into fma(a, b, -(a b)), which is
+  // the product's rounding error and not zero
+  if (!__builtin_isfinite(x)) {
+    *s = *c = __builtin_nan("");
     return;
   }
   const double two_over_pi = 0.6366197723675814;
@@ -85,6 +88,36 @@ LG_HD void sincos_f64(double x, double *s, double *c) {
   else if (q == 1) { *s = cr; *c = -sr; }
   else if (q == 2) { *s = -sr; *c = -cr; }
   else { *s = -cr; *c = sr; }
+}
+
+// sin and cos in FP32 of an FP64 argument (the phase of an FP32 potential step): x = n pi/2 + r
+// reduced in FP64 with the two leading parts of pi/2 above (exact products for |n| < 2^20;
+// the dropped third part is below 1e-14 absolute there), only |r| <= pi/4 rounded to FP32, then
+// Taylor series of sin to r^9 and cos to r^10 in FP32 (truncation < 3e-9 and 1.2e-10), quadrant
+// from n mod 4 taken in FP64.  Within 2 FP32 ulp of the correctly rounded result
+// (tests/test_phase_math.py), and more accurate than rounding x itself to FP32 first.
+LG_HD void sincos_f32(double x, float *s, float *c) {
+  const double two_over_pi = 0.6366197723675814;
+  const double pio2_hi = 1.5707963267341256;
+  const double pio2_mid = 6.077100506303966e-11;
+  const double n = rint(x * two_over_pi);
+  const float r = (float)((x - n * pio2_hi) - n * pio2_mid);
+  const int q = (int)(n - 4.0 * floor(0.25 * n));  // n mod 4 in 0..3
+  const float r2 = r * r;
+  float ps = 2.7557319e-06f;   // 1/9!
+  ps = ps * r2 - 1.9841270e-04f;  // -1/7!
+  ps = ps * r2 + 8.3333333e-03f;  // 1/5!
+  ps = ps * r2 - 1.6666667e-01f;  // -1/3!
+  const float sr = r + (r * r2) * ps;
+  float pc = -2.7557319e-07f;  // -1/10!
+  pc = pc * r2 + 2.4801587e-05f;  // 1/8!
+  pc = pc * r2 - 1.3888889e-03f;  // -1/6!
+  pc = pc * r2 + 4.1666667e-02f;  // 1/4!
+  pc = pc * r2 - 0.5f;
+  const float cr = 1.0f + r2 * pc;
+  const float s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -s0 : s0;
+  *c = ((q + 1) & 2) ? -c0 : c0;
 }
 
 }  // namespace lg

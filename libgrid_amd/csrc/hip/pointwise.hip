@@ -257,10 +257,8 @@ template <> __device__ __forceinline__ double2 pfac<double2>(double cr, double c
 }
 template <> __device__ __forceinline__ float2 pfac<float2>(double cr, double ci, double vr, double vi) {
   const double ar = cr * vr - ci * vi, ai = cr * vi + ci * vr;
-  const double twopi = 6.283185307179586476925286766559;
-  const double red = ai - twopi * rint(ai * (1.0 / twopi));
   float s, c;
-  sincosf((float)red, &s, &c);
+  sincos_f32(ai, &s, &c);
   const float m = expf((float)ar);
   return make_float2(m * c, m * s);
 }
