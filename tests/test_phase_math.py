@@ -39,7 +39,7 @@ int main() {
   std::printf("%g %g %g %g %g %g %g %g\n", es, ec, ee, ew, s0, c0, lg::exp_f64(0.0), lg::exp_f64(-800.0));
   // FP32 sin / cos of an FP64 argument: error in FP32 ulp of the result (ulp of 2^-24 below
   // |result| = 2^-24 scale kept at the FP32 ulp of 1e-3 so that zeros do not dominate), and the
-  // absolute error, against the old form sincosf((float) x reduced mod 2 pi)
+  // absolute error, against FP32 sin / cos of x reduced mod 2 pi in FP64 and rounded to FP32
   double fs = 0, fc = 0, fa = 0, old_a = 0;
   for (int i = 0; i < 2000000; ++i) {
     double x = i < 1000 ? (i - 500) * 0.7853981633974483 * 0.5 : std::uniform_real_distribution<double>(-2000.0, 2000.0)(g);
