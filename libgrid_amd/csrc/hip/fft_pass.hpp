@@ -355,9 +355,9 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   // XCD-aware order on the y axis and on the wide x tiles: workgroup b runs on XCD b % 8, so
   // the first 8 (nb / 8) workgroups are renumbered to give each XCD a contiguous run of tiles
   // (the tail keeps its number).  At 512^3 FP64 the y pass went from 916 to 780 us and the wide
-  // x kinetic pass from 984-988 to 953-971 us (two boxes, alternating runs).  The other x passes (256^3 FP64 kinetic: 93.0-93.3 -> 95.8-96.3 us;
-  // 512^3 FP32: 648 -> 654 us) and the contiguous axis were slower in this order, so they keep
-  // the hardware order.
+  // x kinetic pass from 984-988 to 953-971 us (two boxes, alternating runs).  The other x passes
+  // (256^3 FP64 kinetic: 93.0-93.3 -> 95.8-96.3 us; 512^3 FP32: 648 -> 654 us) and the
+  // contiguous axis were slower in this order, so they keep the hardware order.
   long blk = blockIdx.x;
   if constexpr (!CONTIG) {
     if (a.axis == 1 || WIDE) {
