@@ -23,14 +23,17 @@
 
 namespace lg {
 
+// Branch-free, so that a thread's many evaluations interleave: arguments beyond the range are
+// clamped by selects (a NaN fails both comparisons and stays NaN), and exp(709.79) overflows
+// to inf and exp(-746) underflows to 0 in the final ldexp as the exact values would.
 LG_HD double exp_f64(double x) {
-  if (!(x == x)) return x;
-  if (x > 709.782712893384) return HUGE_VAL;
-  if (x < -745.1332191019412) return 0.0;
+  x = x > 709.79 ? 709.79 : x;
+  x = x < -746.0 ? -746.0 : x;
   const double inv_ln2 = 1.4426950408889634;
   const double ln2_hi = 0.6931471806019545;      // 32 significant bits
   const double ln2_lo = -4.2009150726810846e-11; // ln 2 - ln2_hi
   const double k = rint(x * inv_ln2);
+  const int ki = k == k ? (int)k : 0;  // |k| <= 1077 here; a NaN k is not converted
   const double r = (x - k * ln2_hi) - k * ln2_lo;
   double p = 1.6059043836821613e-10;            // 1/13!
   p = p * r + 2.08767569878681e-09;             // 1/12!
@@ -46,17 +49,16 @@ LG_HD double exp_f64(double x) {
   p = p * r + 0.5;
   p = p * r + 1.0;
   p = p * r + 1.0;
-  return ldexp(p, (int)k);
+  return ldexp(p, ki);
 }
 
-LG_HD void sincos_f64(double x, double *s, double *c) {
-  // inf or nan, by a class test: the form x - x != 0 breaks when x is a product inlined from
-  // the caller, since the compiler may contract (a b) - (a b) into fma(a, b, -(a b)), which is
-  // the product's rounding error and not zero
-  if (!__builtin_isfinite(x)) {
-    *s = *c = __builtin_nan("");
-    return;
-  }
+// Branch-free like exp_f64: the quadrant is applied by selects, and an inf or NaN argument is
+// replaced by 0 for the evaluation and gives NaN at the end.  That test is a class test: the
+// form x - x != 0 breaks when x is a product inlined from the caller, since the compiler may
+// contract (a b) - (a b) into fma(a, b, -(a b)), which is the product's rounding error.
+LG_HD void sincos_f64(double x_in, double *s, double *c) {
+  const bool finite = __builtin_isfinite(x_in);
+  const double x = finite ? x_in : 0.0;
   const double two_over_pi = 0.6366197723675814;
   const double pio2_hi = 1.5707963267341256;      // 33 significant bits
   const double pio2_mid = 6.077100506303966e-11;  // 33 significant bits
@@ -83,11 +85,11 @@ LG_HD void sincos_f64(double x, double *s, double *c) {
   pc = pc * r2 + 0.041666666666666664;   // 1/4!
   pc = pc * r2 - 0.5;
   const double cr = 1.0 + r2 * pc;
-  const long long q = (long long)n & 3;
-  if (q == 0) { *s = sr; *c = cr; }
-  else if (q == 1) { *s = cr; *c = -sr; }
-  else if (q == 2) { *s = -sr; *c = -cr; }
-  else { *s = -cr; *c = sr; }
+  const int q = (int)(n - 4.0 * floor(0.25 * n));  // n mod 4 in 0..3, exact for |n| < 2^52
+  const double s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
+  const double nan = __builtin_nan("");
+  *s = finite ? ((q & 2) ? -s0 : s0) : nan;
+  *c = finite ? (((q + 1) & 2) ? -c0 : c0) : nan;
 }
 
 // sin and cos in FP32 of an FP64 argument (the phase of an FP32 potential step): x = n pi/2 + r
@@ -102,7 +104,8 @@ LG_HD void sincos_f32(double x, float *s, float *c) {
   const double pio2_mid = 6.077100506303966e-11;
   const double n = rint(x * two_over_pi);
   const float r = (float)((x - n * pio2_hi) - n * pio2_mid);
-  const int q = (int)(n - 4.0 * floor(0.25 * n));  // n mod 4 in 0..3
+  const double nq = n - 4.0 * floor(0.25 * n);  // n mod 4 in 0..3, NaN for an inf or NaN x
+  const int q = nq == nq ? (int)nq : 0;          // (r is NaN then, and so are sin and cos)
   const float r2 = r * r;
   float ps = 2.7557319e-06f;   // 1/9!
   ps = ps * r2 - 1.9841270e-04f;  // -1/7!

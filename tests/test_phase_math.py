@@ -69,9 +69,24 @@ int main() {
     lg::sincos_f64(ci * vr, &s, &c);
     pe = std::fmax(pe, std::fabs(s * s + c * c - 1.0));
   }
-  double sn, cn;
+  double sn, cn, sm, cm;
   lg::sincos_f64(INFINITY, &sn, &cn);
-  std::printf("%g %d %d\n", pe, (int)std::isnan(sn), (int)std::isnan(cn));
+  lg::sincos_f64(std::nan(""), &sm, &cm);
+  std::printf("%g %d %d\n", pe, (int)(std::isnan(sn) && std::isnan(sm)), (int)(std::isnan(cn) && std::isnan(cm)));
+  // exp at and beyond the ends of its range, and in the subnormal results
+  double sub = 0;
+  for (int i = 0; i < 20000; ++i) {
+    double y = std::uniform_real_distribution<double>(-745.0, -708.5)(g);
+    // error in units of the smallest subnormal, the spacing of every result here
+    sub = std::fmax(sub, (double)(std::fabs((long double)lg::exp_f64(y) - expl((long double)y)) / 4.9406564584124654e-324L));
+  }
+  float fsn, fcn;
+  lg::sincos_f32(std::nan(""), &fsn, &fcn);
+  std::printf("%g %d %d %d %d %d %d %d\n", sub, (int)std::isnan(lg::exp_f64(std::nan(""))),
+              (int)std::isinf(lg::exp_f64(709.79)), (int)std::isinf(lg::exp_f64(1e6)),
+              (int)(lg::exp_f64(-745.2) == 0.0 && lg::exp_f64(-1e6) == 0.0),
+              (int)(std::isfinite(lg::exp_f64(709.78)) && lg::exp_f64(709.78) > 1.7e308),
+              (int)(lg::exp_f64(-745.1) > 0.0), (int)(std::isnan(fsn) && std::isnan(fcn)));
   float sz, cz, sq, cq;
   lg::sincos_f32(0.0, &sz, &cz);
   lg::sincos_f32(-3.0 * 1.5707963267948966, &sq, &cq);
@@ -92,7 +107,10 @@ def test_compact_exp_sincos_accuracy():
     es, ec, ee, ew, s0, c0, e0, eunder = map(float, out[:8])
     pe, nan_s, nan_c = map(float, out[8:11])
     assert pe < 1e-15 and nan_s == 1 and nan_c == 1, (pe, nan_s, nan_c)
-    fs, fc, fa, old_a, sz, cz, sq, cq = map(float, out[11:])
+    sub, *flags = map(float, out[11:19])
+    assert sub <= 1.0, sub  # subnormal results within one spacing of exp
+    assert all(f == 1 for f in flags), flags  # NaN, overflow, underflow, last finite, first subnormal
+    fs, fc, fa, old_a, sz, cz, sq, cq = map(float, out[19:])
     # FP32 sin / cos of an FP64 argument (the FP32 potential step's phase)
     assert fs < 2.0 and fc < 2.0, (fs, fc)
     assert fa < 1.2e-7 and fa < old_a, (fa, old_a)
