@@ -407,9 +407,24 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
     for (int i = tid; i < L; i += NT) kt_lds[i] = ka[i];
   }
 
+  // Loads by every thread, zeroed by a select for a thread past the last pencil (which reads
+  // elements of pencil 0, base 0, in bounds), instead of a branch around each load.  Measured
+  // against the branch form: contiguous passes faster (256^3 FP64 start-of-step pass 101.7-102.0
+  // -> 100.3-100.5 us, potential pass 114.7-115.0 -> 113.1-113.2 us), 512-point plain transforms
+  // on y too (FP64 779 -> 766 us, FP32 399 -> 394 us), but the wide FP64 x kinetic pass slower
+  // (966-970 -> 995-1004 us) and the 256-point y pass a little slower (81.9-82.0 -> 82.4-82.6
+  // us), so those keep the branches.
+  constexpr bool SELECT_LOADS = CONTIG || (KIND == KIND_FY && L == 512 && !WIDE);
   C v[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = active ? widen<C>(bload<S>(rin, e0 * CS, (unsigned)j * estep * CS)) : mk<C>(0, 0);
+  for (int j = 0; j < R; ++j) {
+    if constexpr (SELECT_LOADS) {
+      const C x = widen<C>(bload<S>(rin, e0 * CS, (unsigned)j * estep * CS));
+      v[j] = active ? x : mk<C>(0, 0);
+    } else {
+      v[j] = active ? widen<C>(bload<S>(rin, e0 * CS, (unsigned)j * estep * CS)) : mk<C>(0, 0);
+    }
+  }
   if constexpr (KT_LDS && TPP == 1) __syncthreads();
 
   double acc[HGRID_RED_NCHAN] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -430,17 +445,41 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   double vxp[KIND == KIND_PRE ? R : 1];
   if constexpr (KIND == KIND_PRE) {
 #pragma unroll
-    for (int j = 0; j < R; ++j) vxp[j] = (a.vext && active) ? bload_f64(rvx, (e0 + (unsigned)j * estep) * 8u, 0) : 0.0;
+    for (int j = 0; j < R; ++j) {
+      // a null vext has a zero-length buffer, whose loads return 0
+      const double x = bload_f64(rvx, (e0 + (unsigned)j * estep) * 8u, 0);
+      vxp[j] = active ? x : 0.0;
+    }
   }
   auto pre_ops = [&]() {
     const auto rrho = make_rsrc(a.rho_out, (unsigned long)ntot * 8);
+    if constexpr (KIND == KIND_PRE) {
+      // vext is held: the stores go under one branch, the sums need none (an inactive thread
+      // holds zeros)
+      if (active && (fl & (HGRID_PF_PRE_VEFF | HGRID_PF_PRE_RHO))) {
+        const bool veff = fl & HGRID_PF_PRE_VEFF;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const double d = cnorm2(v[j]);
+          bstore_f64(veff ? vxp[j] + a.g * d : d, rrho, (e0 + (unsigned)j * estep) * 8u, 0);
+        }
+      }
+      if (fl & HGRID_PF_PRE_RED) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const double d = cnorm2(v[j]);
+          acc[HGRID_RED_NORM] += d;
+          acc[HGRID_RED_QUART] += d * d;
+          acc[HGRID_RED_VEXT] += vxp[j] * d;
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const unsigned e = e0 + (unsigned)j * estep;
       const double d = cnorm2(v[j]);
-      double vx;
-      if constexpr (KIND == KIND_PRE) vx = vxp[j];
-      else vx = (a.vext && active) ? bload_f64(rvx, e * 8u, 0) : 0.0;
+      const double vx = active ? bload_f64(rvx, e * 8u, 0) : 0.0;
       if (active) {
         if (fl & HGRID_PF_PRE_VEFF) bstore_f64(vx + a.g * d, rrho, e * 8u, 0);
         else if (fl & HGRID_PF_PRE_RHO) bstore_f64(d, rrho, e * 8u, 0);
