@@ -411,7 +411,7 @@ __attribute__((amdgpu_waves_per_eu(waves_hint<C, L, CONTIG, KIND>(), waves_cap<C
   // elements of pencil 0, base 0, in bounds), instead of a branch around each load.  Measured
   // against the branch form: contiguous passes faster (256^3 FP64 start-of-step pass 101.7-102.0
   // -> 100.3-100.5 us, potential pass 114.7-115.0 -> 113.1-113.2 us), 512-point plain transforms
-  // on y too (FP64 779 -> 766 us, FP32 399 -> 394 us), but the wide FP64 x kinetic pass slower
+  // too (on y: FP64 779 -> 766 us, FP32 399 -> 394 us), but the wide FP64 x kinetic pass slower
   // (966-970 -> 995-1004 us) and the 256-point y pass a little slower (81.9-82.0 -> 82.4-82.6
   // us), so those keep the branches.
   constexpr bool SELECT_LOADS = CONTIG || (KIND == KIND_FY && L == 512 && !WIDE);
